@@ -1,0 +1,181 @@
+/*
+ * snapmrf.h -- C ABI of the MI355X (gfx950) snapMRF dictionary-generation and
+ * template-matching path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns 0
+ * on success and a negative MRF_E_* code on failure (the message is kept in a
+ * thread-local buffer read by mrf_last_error()).  Nothing here exits,
+ * aborts or blocks on stdin, unlike the reference's gpuAssert/err paths
+ * (src/functions.cu:37-74).  Device pointers are caller-owned.
+ *
+ * Reference interface each entry point replaces is cited as file:line of the
+ * snapMRF source tree (src/...).
+ */
+#ifndef SNAPMRF_H
+#define SNAPMRF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* complex64, layout-identical to CUDA/HIP float2 and std::complex<float>
+ * (src/float2math.h) */
+typedef struct { float x, y; } mrf_complex;
+
+/* error codes */
+#define MRF_OK 0
+#define MRF_E_ARG (-1)       /* invalid argument / size */
+#define MRF_E_IO (-2)        /* file open/read/write failure */
+#define MRF_E_FORMAT (-3)    /* malformed CSV / RA content */
+#define MRF_E_ECHO (-4)      /* unknown echo type (src/functions.cu:1310-1311) */
+#define MRF_E_GPU (-5)       /* HIP runtime error or no device */
+#define MRF_E_NOMEM (-6)     /* allocation failure */
+#define MRF_E_UNSUPPORTED (-7)
+
+/* Thread-local message of the last failing call on this thread. */
+const char *mrf_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host helpers (drop-in counterparts of the reference's host functions)      */
+/* ------------------------------------------------------------------------ */
+
+/* Number of '\n' in the file (src/functions.cu:100-120 csv_dim); the
+ * sequence length is csv_dim - 1.  Returns (size_t)-1 on open failure. */
+size_t mrf_csv_dim(const char *filepath);
+
+/* Reads nreps rows of the 4-column rfpulse CSV into h_mrf[4][nreps] (FA |
+ * phase | TR | TE), reading 100-byte fgets chunks and skipping the first two
+ * chunks exactly as src/functions.cu:122-158 csv_read does.  Rows missing
+ * because of that skip are zero-filled and reported in mrf_last_error()
+ * (return value 1).  Returns MRF_E_FORMAT for a row with fewer than 4 fields. */
+int mrf_csv_read(float *h_mrf, size_t nreps, const char *filepath);
+
+/* Grid strings "a", "a:b" (step 1), "a:s:b", joined by '+'
+ * (src/functions.cu:196-353 parse_range/parse_length/parse_params), float32
+ * arithmetic, including the reference's dedup and end bookkeeping.
+ * mrf_parse_length returns 0 on a malformed string. */
+size_t mrf_parse_length(const char *str);
+int mrf_parse_params(float *h_p, const char *str, size_t len);
+
+/* logspace of the -N mode (src/functions.cu:269-304).  For min <= 0 the
+ * reference writes one element past len; h_p must hold len + 1 floats. */
+void mrf_logspace(float *h_p, float min, float max, size_t len);
+
+/* Atoms with T1 > T2, times l_b0 * l_b1 (src/functions.cu:393-414). */
+size_t mrf_compute_natoms(const float *h_t1, const float *h_t2, size_t l_t1,
+                          size_t l_t2, size_t l_b0, size_t l_b1);
+
+/* params[A][nparams] in the reference atom order: T1 fastest, then T2, B0,
+ * B1 (src/functions.cu:355-391 trans_params). */
+int mrf_trans_params(float *h_params, const float *h_t1, const float *h_t2,
+                     const float *h_b0, const float *h_b1, size_t l_t1,
+                     size_t l_t2, size_t l_b0, size_t l_b1, size_t NATOMS,
+                     int nparams);
+
+/* The -N hard-coded grid of src/mrf.cu:179-203 (l_b1=11, l_b0=21,
+ * l_t1=l_t2=round(sqrt(N/231))).  Fills the lengths; if the four arrays are
+ * non-NULL they must hold l_t1, l_t2, l_b0 + 1 and l_b1 floats. */
+int mrf_grid_natoms_mode(size_t N, size_t *l_t1, size_t *l_t2, size_t *l_b0,
+                         size_t *l_b1, float *h_t1, float *h_t2, float *h_b0,
+                         float *h_b1);
+
+/* RA files (src/ra.h:20-74, src/ra.cu:23-105).  Same bytes; headers are
+ * validated and writes larger than 2 GiB are correct. */
+typedef struct {
+    uint64_t flags, eltype, elbyte, size, ndims;
+    uint64_t *dims;
+    uint8_t *data;
+} mrf_ra_t;
+int mrf_ra_read(mrf_ra_t *a, const char *path);
+int mrf_ra_write(const mrf_ra_t *a, const char *path);
+void mrf_ra_free(mrf_ra_t *a);
+
+/* ------------------------------------------------------------------------ */
+/* Dictionary generation                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Drop-in for MRF_dict (src/functions.cu:1224-1315, declared
+ * src/functions.h:115-117).  d_atoms [NATOMS][nreps] complex64 (device, RA
+ * dims (nreps, NATOMS)); d_params [NATOMS][nparams] (device); h_mrf host
+ * [4][NREPS] csv columns of which the first nreps rows are used.
+ * echo_type: epg_ssfp, roa_ssfp, epg_sr, epg_se, epg_tse1, epg_tse2,
+ * epg_fisp.  Semantics follow the reference: csv unit transform and L2
+ * normalisation for epg_ssfp/roa_ssfp only; roa_ssfp and the sr/se/tse/fisp
+ * programs fill the first natoms atoms only.  Runs on the null stream and
+ * returns after completion. */
+int mrf_dict(mrf_complex *d_atoms, const float *d_params, const float *h_mrf,
+             size_t nreps, size_t NREPS, size_t NATOMS, size_t natoms,
+             int nparams, int nstates, const char *echo_type);
+
+/* Options of the extended entry points.  Zero-initialise, then set. */
+typedef struct {
+    int normalize;        /* -1: reference default for the echo type, 0/1 force */
+    int n_iso;            /* roa_ssfp isochromats per atom (<=1: reference) */
+    float iso_spread_hz;  /* roa_ssfp off-resonance band of the isochromats */
+    int reserved[8];
+} mrf_dict_opts;
+
+/* Extended dictionary generation on a caller stream (hipStream_t passed as
+ * void*; NULL = null stream).  Asynchronous with respect to the host.
+ * d_norm2 (optional, [NATOMS] double) receives each atom's squared L2 norm
+ * before normalisation. */
+int mrf_dict_ex(void *stream, const char *echo_type, const float *h_mrf,
+                size_t nreps, size_t NREPS, const float *d_params,
+                size_t NATOMS, size_t natoms, int nstates,
+                const mrf_dict_opts *opts, mrf_complex *d_atoms,
+                double *d_norm2);
+
+/* ------------------------------------------------------------------------ */
+/* Matching                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Drop-in for MRF_match (src/functions.cu:1493-1584, declared
+ * src/functions.h:132-134).  d_maps [nvoxels][nmaps] (device) receives
+ * (T1, T2, B0, B1, PD) of the best atom per voxel; h_img host
+ * [nvoxels][NREPS] complex64 of which nreps frames are matched (as in the
+ * reference, the voxel stride is NREPS); d_atoms/d_params as produced by
+ * mrf_dict.  in_b1map: path of a float32 RA B1 map with nvoxels entries, or
+ * NULL for the max-PD merge.  Synchronous. */
+int mrf_match(float *d_maps, const mrf_complex *h_img,
+              const mrf_complex *d_atoms, const float *d_params, size_t NREPS,
+              size_t nreps, size_t NATOMS, size_t natoms, size_t nvoxels,
+              int nparams, int nmaps, const char *in_b1map);
+
+/* Matching precisions of mrf_match_ex */
+#define MRF_MATCH_FP32 0        /* complex64 dictionary, FP32 MFMA */
+#define MRF_MATCH_BF16_TOP8 1   /* BF16 MFMA + top-8 + FP32 rescore */
+#define MRF_MATCH_SPLIT3 2      /* split-BF16 3-pass + top-8 + FP32 rescore */
+
+/* Extended matching, all device-resident, on a caller stream.
+ *   d_atoms   [NATOMS][T] complex64 (used by FP32 and for the rescore)
+ *   d_lowp    [NATOMS][T] bf16x2 hi copy (BF16/SPLIT3 modes; from
+ *             mrf_lowp_prepare), d_lowp_lo the lo residual (SPLIT3)
+ *   d_sig     [V][T] complex64 signals
+ *   d_b1map   [V] float B1 map or NULL (max-PD merge over all slices)
+ *   d_maps    [V][5] (T1, T2, B0, B1, PD); d_idx optional [V] int32 global
+ *             atom index of the match.
+ * B1-map semantics: each voxel is matched only against the slice k whose B1
+ * minimises |b1map - B1_k| (strict <, first k on ties), which equals the
+ * reference's per-slice argmax followed by merge_maps(b1map)
+ * (src/functions.cu:1351-1384) because every atom of slice k carries B1_k. */
+int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
+                 const void *d_lowp, const void *d_lowp_lo,
+                 const float *d_params, size_t T, size_t NATOMS,
+                 size_t natoms, const mrf_complex *d_sig, size_t V,
+                 const float *d_b1map, float *d_maps, int32_t *d_idx);
+
+/* Builds the BF16 copies of a complex64 dictionary: d_hi = bf16(a),
+ * d_lo = bf16(a - hi) (optional).  Each is [NATOMS][T] x 2 bf16. */
+int mrf_lowp_prepare(void *stream, const mrf_complex *d_atoms, size_t T,
+                     size_t NATOMS, void *d_hi, void *d_lo);
+
+/* Device count visible to the library (0 when no GPU). */
+int mrf_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,277 @@
+"""snapmrf_amd -- MI355X (gfx950) dictionary generation and template matching
+for MR fingerprinting, a drop-in for snapMRF's hot path.
+
+The compute path is the in-tree C-ABI library ``libsnapmrf.so`` (HIP kernels +
+C++ host code, declared in ``include/snapmrf.h``).  This module is a thin
+ctypes binding over it that mirrors the reference's interface
+(``MRF_dict`` / ``MRF_match`` of src/functions.h:115-134, same argument names
+and meaning) for torch device tensors.  There is no CPU fallback: if the
+library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsnapmrf.so")
+CLI_PATH = os.path.join(_HERE, "bin", "mrf")
+
+MRF_MATCH_FP32 = 0
+MRF_MATCH_BF16_TOP8 = 1
+MRF_MATCH_SPLIT3 = 2
+
+ECHO_TYPES = ("epg_ssfp", "roa_ssfp", "epg_sr", "epg_se", "epg_tse1", "epg_tse2", "epg_fisp")
+
+# every symbol include/snapmrf.h declares
+EXPORTS = (
+    "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
+    "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
+    "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
+    "mrf_match_ex", "mrf_lowp_prepare", "mrf_device_count",
+)
+
+
+class MRFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class MrfRa(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint64), ("eltype", ctypes.c_uint64),
+                ("elbyte", ctypes.c_uint64), ("size", ctypes.c_uint64),
+                ("ndims", ctypes.c_uint64), ("dims", ctypes.POINTER(ctypes.c_uint64)),
+                ("data", ctypes.POINTER(ctypes.c_uint8))]
+
+
+class MrfDictOpts(ctypes.Structure):
+    _fields_ = [("normalize", ctypes.c_int), ("n_iso", ctypes.c_int),
+                ("iso_spread_hz", ctypes.c_float), ("reserved", ctypes.c_int * 8)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libsnapmrf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m snapmrf_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, S, I, F = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_float
+    cs = ctypes.c_char_p
+    sig = {
+        "mrf_last_error": ([], cs),
+        "mrf_csv_dim": ([cs], S),
+        "mrf_csv_read": ([P, S, cs], I),
+        "mrf_parse_length": ([cs], S),
+        "mrf_parse_params": ([P, cs, S], I),
+        "mrf_logspace": ([P, F, F, S], None),
+        "mrf_compute_natoms": ([P, P, S, S, S, S], S),
+        "mrf_trans_params": ([P, P, P, P, P, S, S, S, S, S, I], I),
+        "mrf_grid_natoms_mode": ([S, P, P, P, P, P, P, P, P], I),
+        "mrf_ra_read": ([ctypes.POINTER(MrfRa), cs], I),
+        "mrf_ra_write": ([ctypes.POINTER(MrfRa), cs], I),
+        "mrf_ra_free": ([ctypes.POINTER(MrfRa)], None),
+        "mrf_dict": ([P, P, P, S, S, S, S, I, I, cs], I),
+        "mrf_dict_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, P], I),
+        "mrf_match": ([P, P, P, P, S, S, S, S, S, I, I, cs], I),
+        "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
+        "mrf_lowp_prepare": ([P, P, S, S, P, P], I),
+        "mrf_device_count": ([], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().mrf_last_error().decode(errors="replace")
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise MRFError(rc, last_error())
+    return rc
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    if hasattr(t, "ctypes"):
+        return t.ctypes.data
+    return int(t)
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _enc(s: str | None):
+    return None if s is None else s.encode()
+
+
+# ---------------------------------------------------------------- host ----
+
+def csv_dim(path: str) -> int:
+    n = lib().mrf_csv_dim(_enc(path))
+    if n == ctypes.c_size_t(-1).value:
+        raise MRFError(-2, last_error())
+    return n
+
+
+def csv_read(path: str, nreps: int):
+    """Returns float32 [4, nreps] (FA | phase | TR | TE) and the warning text
+    ('' when every row was read)."""
+    import numpy as np
+    out = np.zeros((4, nreps), dtype=np.float32)
+    rc = check(lib().mrf_csv_read(out.ctypes.data, nreps, _enc(path)))
+    return out, (last_error() if rc > 0 else "")
+
+
+def parse_grid(s: str):
+    import numpy as np
+    n = lib().mrf_parse_length(_enc(s))
+    if n == 0:
+        raise MRFError(-1, last_error())
+    out = np.zeros(n, dtype=np.float32)
+    check(lib().mrf_parse_params(out.ctypes.data, _enc(s), n))
+    return out
+
+
+def logspace(mn: float, mx: float, n: int):
+    import numpy as np
+    out = np.zeros(n + 1, dtype=np.float32)
+    lib().mrf_logspace(out.ctypes.data, mn, mx, n)
+    return out[:n]
+
+
+def build_params(T1: str, T2: str, B0: str, B1: str):
+    """Grid strings -> (params float32 [NATOMS, 4], NATOMS, natoms), reference
+    order (T1 fastest, then T2, B0, B1)."""
+    import numpy as np
+    t1, t2, b0, b1 = (parse_grid(x) for x in (T1, T2, B0, B1))
+    return params_from_axes(t1, t2, b0, b1)
+
+
+def params_from_axes(t1, t2, b0, b1):
+    import numpy as np
+    t1, t2, b0, b1 = (np.ascontiguousarray(x, dtype=np.float32) for x in (t1, t2, b0, b1))
+    L = lib()
+    NATOMS = L.mrf_compute_natoms(t1.ctypes.data, t2.ctypes.data, len(t1), len(t2), len(b0), len(b1))
+    if NATOMS == 0:
+        raise MRFError(-1, "Number of atoms is 0")
+    params = np.zeros((NATOMS, 4), dtype=np.float32)
+    check(L.mrf_trans_params(params.ctypes.data, t1.ctypes.data, t2.ctypes.data, b0.ctypes.data,
+                             b1.ctypes.data, len(t1), len(t2), len(b0), len(b1), NATOMS, 4))
+    return params, NATOMS, NATOMS // len(b1)
+
+
+def grid_natoms_mode(N: int):
+    """The -N grid of the reference CLI -> (t1, t2, b0, b1) float32 arrays."""
+    import numpy as np
+    L = lib()
+    lens = [ctypes.c_size_t() for _ in range(4)]
+    check(L.mrf_grid_natoms_mode(N, *[ctypes.byref(x) for x in lens], None, None, None, None))
+    arrs = [np.zeros(x.value + 1, dtype=np.float32) for x in lens]
+    check(L.mrf_grid_natoms_mode(N, *[ctypes.byref(x) for x in lens], *[a.ctypes.data for a in arrs]))
+    return tuple(a[:x.value] for a, x in zip(arrs, lens))
+
+
+def ra_read(path: str):
+    """Reads an RA file through the library -> (numpy array shaped dims[::-1], header dict)."""
+    import numpy as np
+    ra = MrfRa()
+    check(lib().mrf_ra_read(ctypes.byref(ra), _enc(path)))
+    try:
+        dims = [ra.dims[i] for i in range(ra.ndims)]
+        dt = {(3, 4): np.float32, (3, 8): np.float64, (4, 8): np.complex64,
+              (4, 16): np.complex128, (1, 4): np.int32, (1, 8): np.int64,
+              (2, 4): np.uint32, (2, 8): np.uint64}.get((ra.eltype, ra.elbyte), np.uint8)
+        raw = ctypes.string_at(ra.data, ra.size)
+        arr = np.frombuffer(raw, dtype=dt).copy()
+        if dt is not np.uint8:
+            arr = arr.reshape(dims[::-1])
+        hdr = dict(flags=ra.flags, eltype=ra.eltype, elbyte=ra.elbyte, size=ra.size, dims=dims)
+    finally:
+        lib().mrf_ra_free(ctypes.byref(ra))
+    return arr, hdr
+
+
+def ra_write(path: str, arr, dims=None) -> None:
+    import numpy as np
+    arr = np.ascontiguousarray(arr)
+    kind = {np.dtype(np.float32): (3, 4), np.dtype(np.float64): (3, 8),
+            np.dtype(np.complex64): (4, 8), np.dtype(np.complex128): (4, 16),
+            np.dtype(np.int32): (1, 4), np.dtype(np.int64): (1, 8)}[arr.dtype]
+    dims = list(arr.shape[::-1]) if dims is None else list(dims)
+    d = (ctypes.c_uint64 * len(dims))(*dims)
+    ra = MrfRa(0, kind[0], kind[1], arr.nbytes, len(dims),
+               ctypes.cast(d, ctypes.POINTER(ctypes.c_uint64)),
+               ctypes.cast(arr.ctypes.data, ctypes.POINTER(ctypes.c_uint8)))
+    check(lib().mrf_ra_write(ctypes.byref(ra), _enc(path)))
+
+
+# ---------------------------------------------------------------- GPU ----
+
+def device_count() -> int:
+    return int(lib().mrf_device_count())
+
+
+def MRF_dict(d_atoms, d_params, h_mrf, nreps: int, NREPS: int, NATOMS: int, natoms: int,
+             nparams: int, nstates: int, echo_type: str) -> None:
+    """Mirror of MRF_dict (src/functions.cu:1224).  d_atoms: device complex64
+    [NATOMS, nreps]; d_params device float32 [NATOMS, 4]; h_mrf host float32
+    [4, NREPS] (numpy).  Synchronous."""
+    import numpy as np
+    h = np.ascontiguousarray(h_mrf, dtype=np.float32)
+    check(lib().mrf_dict(_ptr(d_atoms), _ptr(d_params), h.ctypes.data, nreps, NREPS, NATOMS,
+                         natoms, nparams, nstates, _enc(echo_type)))
+
+
+def dict_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int, nstates: int,
+            d_atoms, d_norm2=None, normalize: int = -1, n_iso: int = 1, iso_spread_hz: float = 0.0,
+            generic: bool = False, stream=None) -> None:
+    """Extended dictionary generation on a stream (asynchronous)."""
+    import numpy as np
+    h = np.ascontiguousarray(h_mrf, dtype=np.float32)
+    opts = MrfDictOpts()
+    opts.normalize = normalize
+    opts.n_iso = n_iso
+    opts.iso_spread_hz = iso_spread_hz
+    opts.reserved[0] = 1 if generic else 0
+    check(lib().mrf_dict_ex(_stream_ptr(stream), _enc(echo_type), h.ctypes.data, nreps, h.shape[1],
+                            _ptr(d_params), NATOMS, natoms, nstates, ctypes.byref(opts),
+                            _ptr(d_atoms), _ptr(d_norm2)))
+
+
+def MRF_match(d_maps, h_img, d_atoms, d_params, NREPS: int, nreps: int, NATOMS: int,
+              natoms: int, nvoxels: int, nparams: int, nmaps: int, in_b1map: str | None) -> None:
+    """Mirror of MRF_match (src/functions.cu:1493).  h_img host complex64
+    [nvoxels, NREPS] (numpy); d_maps device float32 [nvoxels, 5]."""
+    import numpy as np
+    h = np.ascontiguousarray(h_img, dtype=np.complex64)
+    check(lib().mrf_match(_ptr(d_maps), h.ctypes.data, _ptr(d_atoms), _ptr(d_params), NREPS,
+                          nreps, NATOMS, natoms, nvoxels, nparams, nmaps, _enc(in_b1map)))
+
+
+def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_sig, V: int,
+             d_maps, d_b1map=None, d_idx=None, d_lowp=None, d_lowp_lo=None, stream=None) -> None:
+    check(lib().mrf_match_ex(_stream_ptr(stream), mode, _ptr(d_atoms), _ptr(d_lowp),
+                             _ptr(d_lowp_lo), _ptr(d_params), T, NATOMS, natoms, _ptr(d_sig), V,
+                             _ptr(d_b1map), _ptr(d_maps), _ptr(d_idx)))
+
+
+def lowp_prepare(d_atoms, T: int, NATOMS: int, d_hi, d_lo=None, stream=None) -> None:
+    check(lib().mrf_lowp_prepare(_stream_ptr(stream), _ptr(d_atoms), T, NATOMS, _ptr(d_hi),
+                                 _ptr(d_lo)))

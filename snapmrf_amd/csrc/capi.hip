@@ -1,0 +1,312 @@
+// capi.hip -- C ABI of the snapMRF dictionary + matching path (host
+// orchestration of the gfx950 kernels).  See include/snapmrf.h.
+#include "snapmrf.h"
+#include "internal.h"
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mrf {
+
+int echo_id(const char *e)
+{
+    // src/functions.cu:1238-1311
+    if (!e) return -1;
+    if (!std::strcmp(e, "epg_ssfp")) return kEpgSsfp;
+    if (!std::strcmp(e, "roa_ssfp")) return kRoaSsfp;
+    if (!std::strcmp(e, "epg_sr")) return kEpgSr;
+    if (!std::strcmp(e, "epg_se")) return kEpgSe;
+    if (!std::strcmp(e, "epg_tse1") || !std::strcmp(e, "epg_tse2")) return kEpgTse;
+    if (!std::strcmp(e, "epg_fisp")) return kEpgFisp;
+    return -1;
+}
+
+namespace {
+
+#define MRF_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return mrf_fail(MRF_E_GPU, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// Per-TR coefficient table, one thread per TR.  For B1-sliced SSFP the flip
+// angles follow the reference's in-place float scaling: FA *= b1_j before
+// slice j and FA *= 1/b1_j after it (src/functions.cu:970-973, 1012-1013).
+__global__ void coef_kernel(const float *seq, int T, const float *params, size_t natoms,
+                            int n_slices, int b1_scale, TrCoef *coef)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= T) return;
+    float fa = seq[n];
+    const float ph = seq[T + n];
+    const float ph_next = (n + 1 < T) ? seq[T + n + 1] : ph;
+    const float tr = seq[2 * T + n], te = seq[3 * T + n];
+    for (int j = 0; j < n_slices; j++) {
+        float b1 = 1.0f;
+        if (b1_scale) {
+            b1 = params[(size_t)j * natoms * 4 + 3];
+            fa = fa * b1;
+        }
+        TrCoef c;
+        const double a = (double)fa;
+        const double ca = cos(a), sa = sin(a);
+        c.ca2 = (float)((1.0 + ca) * 0.5);
+        c.sa2 = (float)((1.0 - ca) * 0.5);
+        c.sa = (float)sa;
+        c.ca = (float)ca;
+        c.zr = (float)cos((double)ph);
+        c.zi = (float)sin((double)ph);
+        const double d = (double)ph - (double)ph_next;
+        c.rr = (float)cos(d);
+        c.ri = (float)sin(d);
+        c.te = te;
+        c.tr = tr;
+        c.tmte = tr - te;
+        c.s2p = (float)sin(2.0 * (double)ph);
+        c.c2p = (float)cos(2.0 * (double)ph);
+        c.pad[0] = c.pad[1] = c.pad[2] = 0.f;
+        coef[(size_t)j * T + n] = c;
+        if (b1_scale) fa = fa * (1.0f / b1);
+    }
+}
+
+// CSV unit transform (src/functions.cu:1240-1261, 1271-1291), float32.
+void prepare_sequence(int echo, const float *h_mrf, size_t nreps, size_t NREPS,
+                      std::vector<float> &out)
+{
+    out.resize(4 * nreps);
+    for (int c = 0; c < 4; c++)
+        std::memcpy(&out[c * nreps], h_mrf + (size_t)c * NREPS, nreps * sizeof(float));
+    if (echo != kEpgSsfp && echo != kRoaSsfp) return;
+    const float fa_scale = (float)(60.0f * M_PI / 180.0f);
+    const float ph_scale = (float)(M_PI / 180.0f);
+    for (size_t i = 0; i < nreps; i++) {
+        out[i] = out[i] * fa_scale;
+        out[nreps + i] = out[nreps + i] * ph_scale;
+        out[2 * nreps + i] = out[2 * nreps + i] + 16.0f;
+        out[3 * nreps + i] = out[3 * nreps + i] + 3.5f;
+    }
+}
+
+int check_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return mrf_fail(MRF_E_GPU, "no HIP device available");
+    return 0;
+}
+
+}  // namespace
+}  // namespace mrf
+
+using namespace mrf;
+
+extern "C" int mrf_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int mrf_dict_ex(void *stream, const char *echo_type, const float *h_mrf,
+                           size_t nreps, size_t NREPS, const float *d_params, size_t NATOMS,
+                           size_t natoms, int nstates, const mrf_dict_opts *opts,
+                           mrf_complex *d_atoms, double *d_norm2)
+{
+    mrf_clear_error();
+    const int echo = echo_id(echo_type);
+    if (echo < 0)
+        return mrf_fail(MRF_E_ECHO, "unknown echo type '%s' (expected epg_ssfp, roa_ssfp, "
+                        "epg_sr, epg_se, epg_tse1, epg_tse2 or epg_fisp)",
+                        echo_type ? echo_type : "(null)");
+    if (!h_mrf || !d_params || !d_atoms) return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (nreps == 0 || nreps > NREPS) return mrf_fail(MRF_E_ARG, "need 0 < nreps <= NREPS");
+    if (nreps > 0x7fffffff) return mrf_fail(MRF_E_ARG, "nreps too large");
+    if (natoms == 0 || NATOMS == 0 || NATOMS % natoms)
+        return mrf_fail(MRF_E_ARG, "NATOMS (%zu) must be a positive multiple of natoms (%zu)",
+                        NATOMS, natoms);
+    if (int rc = check_device()) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int T = (int)nreps;
+    const int n_slices = (echo == kEpgSsfp) ? (int)(NATOMS / natoms) : 1;
+    const size_t n_gen = (size_t)n_slices * natoms;
+    const int normalize = (opts && opts->normalize >= 0)
+                              ? opts->normalize
+                              : (echo == kEpgSsfp || echo == kRoaSsfp);
+
+    std::vector<float> seq;
+    prepare_sequence(echo, h_mrf, nreps, NREPS, seq);
+    bool h_real = true;
+    for (size_t i = 1; i < nreps; i++)
+        if (seq[nreps + i] != seq[nreps]) h_real = false;
+
+    float *d_seq = nullptr;
+    TrCoef *d_coef = nullptr;
+    double *d_n2 = d_norm2;
+    bool own_n2 = false;
+    MRF_HIP(hipMallocAsync((void **)&d_seq, 4 * nreps * sizeof(float), s));
+    MRF_HIP(hipMallocAsync((void **)&d_coef, (size_t)n_slices * T * sizeof(TrCoef), s));
+    if (!d_n2 && normalize) {
+        MRF_HIP(hipMallocAsync((void **)&d_n2, n_gen * sizeof(double), s));
+        own_n2 = true;
+    }
+    MRF_HIP(hipMemcpyAsync(d_seq, seq.data(), 4 * nreps * sizeof(float), hipMemcpyHostToDevice, s));
+    coef_kernel<<<dim3((T + 127) / 128), 128, 0, s>>>(d_seq, T, d_params, natoms, n_slices,
+                                                       echo == kEpgSsfp ? 1 : 0, d_coef);
+    int rc = 0;
+    if (echo == kRoaSsfp) {
+        RoaLaunch l{};
+        l.coef = d_coef;
+        l.params = d_params;
+        l.atoms = reinterpret_cast<float2 *>(d_atoms);
+        l.norm2 = d_n2;
+        l.natoms = natoms;
+        l.T = T;
+        l.n_iso = opts ? opts->n_iso : 1;
+        l.iso_spread_hz = opts ? opts->iso_spread_hz : 0.f;
+        rc = launch_roa(l, s);
+    } else {
+        EpgLaunch l{};
+        l.coef = d_coef;
+        l.params = d_params;
+        l.atoms = reinterpret_cast<float2 *>(d_atoms);
+        l.norm2 = d_n2;
+        l.natoms = natoms;
+        l.n_slices = n_slices;
+        l.T = T;
+        l.w = nstates;
+        l.echo = echo;
+        l.generic = opts && opts->reserved[0] == 1;
+        l.h_real = h_real && echo == kEpgFisp;
+        rc = launch_epg(l, s);
+    }
+    if (!rc && normalize) rc = launch_normalize(reinterpret_cast<float2 *>(d_atoms), d_n2, n_gen, T, s);
+    (void)hipFreeAsync(d_seq, s);
+    (void)hipFreeAsync(d_coef, s);
+    if (own_n2) (void)hipFreeAsync(d_n2, s);
+    return rc;
+}
+
+extern "C" int mrf_dict(mrf_complex *d_atoms, const float *d_params, const float *h_mrf,
+                        size_t nreps, size_t NREPS, size_t NATOMS, size_t natoms, int nparams,
+                        int nstates, const char *echo_type)
+{
+    if (nparams != 4) return mrf_fail(MRF_E_ARG, "nparams must be 4 (T1, T2, B0, B1)");
+    int rc = mrf_dict_ex(nullptr, echo_type, h_mrf, nreps, NREPS, d_params, NATOMS, natoms,
+                         nstates, nullptr, d_atoms, nullptr);
+    if (rc) return rc;
+    MRF_HIP(hipStreamSynchronize(nullptr));
+    MRF_HIP(hipGetLastError());
+    return MRF_OK;
+}
+
+extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
+                            const void *d_lowp, const void *d_lowp_lo, const float *d_params,
+                            size_t T, size_t NATOMS, size_t natoms, const mrf_complex *d_sig,
+                            size_t V, const float *d_b1map, float *d_maps, int32_t *d_idx)
+{
+    mrf_clear_error();
+    if (!d_atoms || !d_params || !d_maps || (V && !d_sig))
+        return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (T == 0 || T > 0x7fffffff) return mrf_fail(MRF_E_ARG, "bad T");
+    if (natoms == 0 || NATOMS == 0 || NATOMS % natoms)
+        return mrf_fail(MRF_E_ARG, "NATOMS (%zu) must be a positive multiple of natoms (%zu)",
+                        NATOMS, natoms);
+    if (mode != MRF_MATCH_FP32 && !d_lowp)
+        return mrf_fail(MRF_E_ARG, "low-precision mode needs the bf16 dictionary");
+    if (mode == MRF_MATCH_SPLIT3 && !d_lowp_lo)
+        return mrf_fail(MRF_E_ARG, "split mode needs the bf16 lo dictionary");
+    if (int rc = check_device()) return rc;
+    if (V == 0) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int n_slices = (int)(NATOMS / natoms);
+    int32_t *d_vs = nullptr;
+    if (d_b1map) {
+        MRF_HIP(hipMallocAsync((void **)&d_vs, V * sizeof(int32_t), s));
+        if (int rc = launch_b1_select(d_b1map, d_params, natoms, n_slices, V, d_vs, s)) return rc;
+    }
+    MatchLaunch l{};
+    l.atoms = reinterpret_cast<const float2 *>(d_atoms);
+    l.lowp_hi = (const uint32_t *)d_lowp;
+    l.lowp_lo = (const uint32_t *)d_lowp_lo;
+    l.params = d_params;
+    l.sig = reinterpret_cast<const float2 *>(d_sig);
+    l.vox_slice = d_vs;
+    l.natoms = natoms;
+    l.n_slices = n_slices;
+    l.T = (int)T;
+    l.V = V;
+    l.mode = mode;
+    l.maps = d_maps;
+    l.idx = d_idx;
+    void *scratch = nullptr;
+    MRF_HIP(hipMallocAsync(&scratch, match_scratch_bytes(l), s));
+    int rc = launch_match(l, scratch, s);
+    (void)hipFreeAsync(scratch, s);
+    if (d_vs) (void)hipFreeAsync(d_vs, s);
+    return rc;
+}
+
+extern "C" int mrf_match(float *d_maps, const mrf_complex *h_img, const mrf_complex *d_atoms,
+                         const float *d_params, size_t NREPS, size_t nreps, size_t NATOMS,
+                         size_t natoms, size_t nvoxels, int nparams, int nmaps,
+                         const char *in_b1map)
+{
+    mrf_clear_error();
+    if (nparams != 4 || nmaps != 5) return mrf_fail(MRF_E_ARG, "need nparams = 4 and nmaps = 5");
+    if (!h_img && nvoxels) return mrf_fail(MRF_E_ARG, "null image");
+    // The reference strides voxels by NREPS but indexes atoms with nreps
+    // (src/functions.cu:1524-1535); it is only meaningful when they agree.
+    if (nreps != NREPS)
+        return mrf_fail(MRF_E_ARG, "image frames (%zu) must equal the dictionary length (%zu)",
+                        nreps, NREPS);
+    if (int rc = check_device()) return rc;
+    std::vector<float> b1;
+    if (in_b1map) {
+        mrf_ra_t ra;
+        if (int rc = mrf_ra_read(&ra, in_b1map)) return rc;
+        size_t n = 1;
+        for (uint64_t d = 0; d < ra.ndims; d++) n *= ra.dims[d];
+        if (ra.eltype != 3 || ra.elbyte != 4 || n < nvoxels) {
+            mrf_ra_free(&ra);
+            return mrf_fail(MRF_E_FORMAT, "%s: need a float32 B1 map with %zu voxels", in_b1map,
+                            nvoxels);
+        }
+        b1.assign((const float *)ra.data, (const float *)ra.data + nvoxels);
+        mrf_ra_free(&ra);
+    }
+    mrf_complex *d_sig = nullptr;
+    float *d_b1 = nullptr;
+    MRF_HIP(hipMalloc((void **)&d_sig, nvoxels * nreps * sizeof(mrf_complex)));
+    int rc = 0;
+    if (hipMemcpy(d_sig, h_img, nvoxels * nreps * sizeof(mrf_complex), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        rc = mrf_fail(MRF_E_GPU, "image upload failed");
+    if (!rc && in_b1map) {
+        if (hipMalloc((void **)&d_b1, nvoxels * sizeof(float)) != hipSuccess ||
+            hipMemcpy(d_b1, b1.data(), nvoxels * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            rc = mrf_fail(MRF_E_GPU, "B1 map upload failed");
+    }
+    if (!rc)
+        rc = mrf_match_ex(nullptr, MRF_MATCH_FP32, d_atoms, nullptr, nullptr, d_params, nreps,
+                          NATOMS, natoms, d_sig, nvoxels, d_b1, d_maps, nullptr);
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess)
+        rc = mrf_fail(MRF_E_GPU, "matching failed: %s", hipGetErrorString(hipGetLastError()));
+    (void)hipFree(d_sig);
+    if (d_b1) (void)hipFree(d_b1);
+    return rc;
+}
+
+extern "C" int mrf_lowp_prepare(void *stream, const mrf_complex *d_atoms, size_t T,
+                                size_t NATOMS, void *d_hi, void *d_lo)
+{
+    mrf_clear_error();
+    if (!d_atoms || !d_hi) return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (int rc = check_device()) return rc;
+    return launch_lowp_prepare(reinterpret_cast<const float2 *>(d_atoms), NATOMS, (int)T,
+                               (uint32_t *)d_hi, (uint32_t *)d_lo, (hipStream_t)stream);
+}

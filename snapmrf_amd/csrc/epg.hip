@@ -1,0 +1,580 @@
+// epg.hip -- EPG and ROA dictionary kernels for gfx950.
+//
+// Layout and mapping
+//   * One atom per lane group of L lanes (L = 1, 2, 4, 8 or 16, inside one
+//     16-lane DPP row).  Lane j of a group holds EPG states k = j*M .. j*M+M-1
+//     (M is a compile-time constant) as F+_k, F-_k, Z_k complex values in
+//     VGPRs.  The gradient shift crosses lanes only at the group-internal
+//     boundaries, with one DPP row_shr / row_shl per value.
+//   * The TR loop runs inside the kernel; the per-TR, per-B1-slice RF
+//     coefficients come from a 64-byte TrCoef record read with uniform
+//     (scalar) loads.
+//   * Output atoms are [slice][atom][t] complex64 (RA dims (T, A), time
+//     fastest), the reference's layout (src/functions.cu:609-617).
+//
+// Fused SSFP/FISP step ("frame" form).  Write the RF of TR n as
+// P_n M0_n P_n^-1 with P_n = diag(e^{i phi_n}, e^{-i phi_n}, 1) and M0_n the
+// Weigel matrix at zero phase (src/functions.cu:483-508).  Relaxation, B0
+// precession and the gradient shift all commute with P_n, so in coordinates
+// x~ = P_n^-1 x the step of TR n is
+//     (P+, P-, Q) = M0_n (F~+, F~-, Z)                       (per state)
+//     F~+_k <- H P+_{k-1},  F~+_0 <- H conj(P-_1),  F~-_k <- conj(H) P-_{k+1},
+//     F~-_{w-1} <- 0,  Z_k <- E1 Q_k (+ 1 - E1 for k = 0)
+//     sample = E2(TE) e^{i theta_TE} e^{i phi_n} P+_0
+// with H = E2(TE) E2(TR-TE) e^{i(theta_TE + theta_TR-TE)} e^{i(phi_n - phi_{n+1})}.
+// This is algebraically the reference's RF, decay(TE), phase(TE), sample,
+// phase(TR-TE), dephase, decay(TR-TE) sequence (src/functions.cu:975-1009),
+// at 26 VALU operations per state instead of the 102 flops written.
+//
+// The SR/SE/TSE programs (and SSFP/FISP when `generic` is requested) run the
+// reference's step order unfused (src/functions.cu:687-933).
+#include "kernels.h"
+#include "snapmrf.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace mrf {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct EpgArgs {
+    const TrCoef *coef;
+    const float *params;
+    float2 *atoms;
+    double *norm2;
+    size_t natoms;
+    int T;
+    int L;          // lanes per atom (power of two, <= 16)
+    int lshift;     // log2(L)
+    int istar;      // truncation index of state w-1 inside its lane, -1 if none
+    int jstar;      // lane of state w-1 inside the group
+    float init_alpha, init_phi, TI;   // initial pulse (float32 radians / ms)
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// conj(a) * b
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Lane i receives lane i-1 of its 16-lane row (row_shr:1); the first lane of
+// each row receives 0.
+__device__ __forceinline__ float shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x111, 0xF, 0xF, true));
+}
+
+// Lane i receives lane i+1 of its 16-lane row (row_shl:1); the last lane of
+// each row receives 0.
+__device__ __forceinline__ float shl1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x101, 0xF, 0xF, true));
+}
+
+// B0 precession angle as the reference forms it: 2*M_PI*B0*t/1000 evaluated
+// in double and handed to a float (src/functions.cu:598-600).
+__device__ __forceinline__ float phase_angle(float B0, float t)
+{
+    return (float)(2.0 * M_PI * (double)B0 * (double)t / 1000.0);
+}
+
+template <int M>
+struct EpgState {
+    float2 fp[M], fm[M], z[M];
+};
+
+// Initial pulse of init_rf_pulse (src/functions.cu:456-481), written to
+// state 0 (lane j == 0) in true coordinates.
+template <int M>
+__device__ __forceinline__ void epg_init(EpgState<M> &s, const EpgArgs &a, int j, float T1,
+                                         float T2)
+{
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        s.fp[i] = make_float2(0.f, 0.f);
+        s.fm[i] = make_float2(0.f, 0.f);
+        s.z[i] = make_float2(0.f, 0.f);
+    }
+    if (j == 0) {
+        float e1 = expf(-a.TI / T1), e2 = expf(-a.TI / T2);
+        float sa, ca, sph, cph;
+        sincosf(a.init_alpha, &sa, &ca);
+        sincosf(a.init_phi, &sph, &cph);
+        float2 tmp = make_float2(e2 * sa * sph, e2 * sa * cph);
+        s.fp[0] = make_float2(tmp.x, -tmp.y);
+        s.fm[0] = tmp;
+        s.z[0] = make_float2(e1 * ca + 1.f - e1, 0.f);
+    }
+}
+
+// Zero F-_{w-1} when the lane grid holds more than w states.
+template <int M>
+__device__ __forceinline__ void epg_truncate(EpgState<M> &s, const EpgArgs &a, int j)
+{
+    if (a.istar >= 0) {
+#pragma unroll
+        for (int i = 0; i < M; i++)
+            if (i == a.istar && j == a.jstar) s.fm[i] = make_float2(0.f, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Fused SSFP / FISP kernel
+// ------------------------------------------------------------------------
+template <int M, bool MULTI, bool PHASE, bool HREAL>
+__global__ __launch_bounds__(kBlock) void epg_fused_kernel(EpgArgs a)
+{
+    const size_t gl = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t atom = MULTI ? (gl >> a.lshift) : gl;
+    const int j = MULTI ? (int)(gl & (size_t)(a.L - 1)) : 0;
+    const int slice = blockIdx.y;
+    const bool valid = atom < a.natoms;
+    const size_t pa = valid ? atom : 0;
+    // SSFP kernels index the slice-0 parameter rows (src/functions.cu:462-463,
+    // 537-538, 596): T1/T2/B0 repeat identically in every B1 slice.
+    const float T1 = a.params[pa * 4 + 0];
+    const float T2 = a.params[pa * 4 + 1];
+    const float B0 = a.params[pa * 4 + 2];
+    const float rT1 = 1.0f / T1, rT2 = 1.0f / T2;
+
+    const TrCoef *cf = a.coef + (size_t)slice * a.T;
+    EpgState<M> s;
+    epg_init(s, a, j, T1, T2);
+    {
+        // to frame coordinates of TR 0
+        const float2 z0 = make_float2(cf[0].zr, cf[0].zi);
+        s.fp[0] = cmulc(z0, s.fp[0]);
+        s.fm[0] = cmul(z0, s.fm[0]);
+    }
+
+    float2 *out = a.atoms + ((size_t)slice * a.natoms + pa) * (size_t)a.T;
+    const bool store = valid && j == 0;
+    double nrm = 0.0;
+
+    for (int n = 0; n < a.T; n++) {
+        const TrCoef c = cf[n];
+        const float ea = expf(-c.te * rT2);
+        const float eb = expf(-c.tmte * rT2);
+        const float E1 = expf(-c.tr * rT1);
+        float2 H, csave;
+        if (PHASE) {
+            float sa_, ca_, sb_, cb_;
+            sincosf(phase_angle(B0, c.te), &sa_, &ca_);
+            sincosf(phase_angle(B0, c.tmte), &sb_, &cb_);
+            const float2 e = cmul(make_float2(ca_, sa_), make_float2(cb_, sb_));
+            const float g = ea * eb;
+            const float2 er = cmul(e, make_float2(c.rr, c.ri));
+            H = make_float2(g * er.x, g * er.y);
+            const float2 ez = cmul(make_float2(ca_, sa_), make_float2(c.zr, c.zi));
+            csave = make_float2(ea * ez.x, ea * ez.y);
+        } else {
+            const float g = ea * eb;
+            H = HREAL ? make_float2(g, 0.f) : make_float2(g * c.rr, g * c.ri);
+            csave = make_float2(ea * c.zr, ea * c.zi);
+        }
+        const float ca2 = c.ca2, sa2 = c.sa2, sa = c.sa;
+        const float eca = E1 * c.ca, eh = E1 * 0.5f * c.sa;
+        const float rec = (j == 0) ? (1.0f - E1) : 0.0f;
+
+        float2 carry = make_float2(0.f, 0.f), p0 = carry, pm0 = carry, fp0 = carry;
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            const float2 A = s.fp[i], B = s.fm[i], C = s.z[i];
+            const float2 pp = make_float2(ca2 * A.x + sa2 * B.x + sa * C.y,
+                                          ca2 * A.y + sa2 * B.y - sa * C.x);
+            const float2 pm = make_float2(sa2 * A.x + ca2 * B.x - sa * C.y,
+                                          sa2 * A.y + ca2 * B.y + sa * C.x);
+            const float dx = A.x - B.x, dy = A.y - B.y;
+            s.z[i] = make_float2(eca * C.x + eh * dy, eca * C.y - eh * dx);
+            if (i == 0) {
+                s.z[0].x += rec;
+                p0 = pp;
+                pm0 = pm;
+            } else {
+                s.fm[i - 1] = HREAL ? make_float2(H.x * pm.x, H.x * pm.y) : cmulc(H, pm);
+                s.fp[i] = carry;
+            }
+            if (i == 1)
+                fp0 = HREAL ? make_float2(H.x * pm.x, -H.x * pm.y)
+                            : cmul(H, make_float2(pm.x, -pm.y));
+            carry = HREAL ? make_float2(H.x * pp.x, H.x * pp.y) : cmul(H, pp);
+        }
+        if (MULTI) {
+            const float2 up = make_float2(shr1(carry.x), shr1(carry.y));
+            const float2 d0 = HREAL ? make_float2(H.x * pm0.x, H.x * pm0.y) : cmulc(H, pm0);
+            const float2 dn = make_float2(shl1(d0.x), shl1(d0.y));
+            s.fp[0] = (j == 0) ? fp0 : up;
+            s.fm[M - 1] = (j == a.L - 1) ? make_float2(0.f, 0.f) : dn;
+        } else {
+            s.fp[0] = fp0;
+            s.fm[M - 1] = make_float2(0.f, 0.f);
+        }
+        epg_truncate(s, a, j);
+
+        const float2 v = cmul(csave, p0);
+        if (store) {
+            out[n] = v;
+            nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+    }
+    if (store && a.norm2) a.norm2[(size_t)slice * a.natoms + atom] = nrm;
+}
+
+// ------------------------------------------------------------------------
+// Reference-order kernel (SR, SE, TSE; SSFP/FISP cross-check)
+// ------------------------------------------------------------------------
+
+// fill_transition_matrix + apply_rf_pulse (src/functions.cu:483-528)
+template <int M>
+__device__ __forceinline__ void g_rf(EpgState<M> &s, const TrCoef &c)
+{
+    const float2 zz = make_float2(c.zr, c.zi);
+    const float2 zi = make_float2(c.zr, -c.zi);
+    const float2 z2 = cmul(zz, zz), zi2 = cmul(zi, zi);
+    const float2 m01 = make_float2(z2.x * c.sa2, z2.y * c.sa2);
+    const float2 m02 = make_float2(zz.y * c.sa, -zz.x * c.sa);          // -i z sa
+    const float2 m10 = make_float2(zi2.x * c.sa2, zi2.y * c.sa2);
+    const float2 m12 = make_float2(-zi.y * c.sa, zi.x * c.sa);          // i zbar sa
+    const float2 m20 = make_float2(0.5f * zi.y * c.sa, -0.5f * zi.x * c.sa);   // -i/2 zbar sa
+    const float2 m21 = make_float2(-0.5f * zz.y * c.sa, 0.5f * zz.x * c.sa);   // i/2 z sa
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        const float2 A = s.fp[i], B = s.fm[i], C = s.z[i];
+        float2 p = cmul(m01, B), q = cmul(m02, C);
+        s.fp[i] = make_float2(c.ca2 * A.x + p.x + q.x, c.ca2 * A.y + p.y + q.y);
+        p = cmul(m10, A);
+        q = cmul(m12, C);
+        s.fm[i] = make_float2(p.x + c.ca2 * B.x + q.x, p.y + c.ca2 * B.y + q.y);
+        p = cmul(m20, A);
+        q = cmul(m21, B);
+        s.z[i] = make_float2(p.x + q.x + c.ca * C.x, p.y + q.y + c.ca * C.y);
+    }
+}
+
+// compute_exp + decay_signal (src/functions.cu:530-583)
+template <int M>
+__device__ __forceinline__ void g_relax(EpgState<M> &s, float t, float T1, float T2, int j)
+{
+    const float e1 = expf(-t / T1), e2 = expf(-t / T2);
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        s.fp[i] = make_float2(e2 * s.fp[i].x, e2 * s.fp[i].y);
+        s.fm[i] = make_float2(e2 * s.fm[i].x, e2 * s.fm[i].y);
+        s.z[i] = make_float2(e1 * s.z[i].x, e1 * s.z[i].y);
+    }
+    if (j == 0) s.z[0].x += 1.f - e1;
+}
+
+// shift_phase (src/functions.cu:585-607)
+template <int M>
+__device__ __forceinline__ void g_phase(EpgState<M> &s, float t, float B0)
+{
+    float sn, cs;
+    sincosf(phase_angle(B0, t), &sn, &cs);
+    const float2 e = make_float2(cs, sn);
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        s.fp[i] = cmul(e, s.fp[i]);
+        s.fm[i] = cmulc(e, s.fm[i]);
+    }
+}
+
+// dephase_gradients_stage1/2 (src/functions.cu:619-668)
+template <int M, bool MULTI>
+__device__ __forceinline__ void g_dephase(EpgState<M> &s, const EpgArgs &a, int j)
+{
+    float2 up = make_float2(0.f, 0.f), dn = up;
+    if (MULTI) {
+        up = make_float2(shr1(s.fp[M - 1].x), shr1(s.fp[M - 1].y));
+        dn = make_float2(shl1(s.fm[0].x), shl1(s.fm[0].y));
+    }
+#pragma unroll
+    for (int i = M - 1; i >= 1; i--) s.fp[i] = s.fp[i - 1];
+#pragma unroll
+    for (int i = 0; i < M - 1; i++) s.fm[i] = s.fm[i + 1];
+    s.fm[M - 1] = (MULTI && j != a.L - 1) ? dn : make_float2(0.f, 0.f);
+    s.fp[0] = (j == 0) ? make_float2(s.fm[0].x, -s.fm[0].y) : up;
+    epg_truncate(s, a, j);
+}
+
+template <int M, bool MULTI, int PROG>
+__global__ __launch_bounds__(kBlock) void epg_generic_kernel(EpgArgs a)
+{
+    const size_t gl = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t atom = MULTI ? (gl >> a.lshift) : gl;
+    const int j = MULTI ? (int)(gl & (size_t)(a.L - 1)) : 0;
+    const int slice = blockIdx.y;
+    const bool valid = atom < a.natoms;
+    const size_t pa = valid ? atom : 0;
+    const float T1 = a.params[pa * 4 + 0];
+    const float T2 = a.params[pa * 4 + 1];
+    const float B0 = a.params[pa * 4 + 2];
+    const TrCoef *cf = a.coef + (size_t)slice * a.T;
+
+    EpgState<M> s;
+    epg_init(s, a, j, T1, T2);
+    if (PROG == kEpgSe || PROG == kEpgTse) {
+        g_dephase<M, MULTI>(s, a, j);
+        g_relax(s, cf[0].te, T1, T2, j);
+    }
+    float2 *out = a.atoms + ((size_t)slice * a.natoms + pa) * (size_t)a.T;
+    const bool store = valid && j == 0;
+    double nrm = 0.0;
+    for (int n = 0; n < a.T; n++) {
+        const TrCoef c = cf[n];
+        g_rf(s, c);
+        if (PROG == kEpgSe || PROG == kEpgTse) g_dephase<M, MULTI>(s, a, j);
+        g_relax(s, c.te, T1, T2, j);
+        if (PROG == kEpgSsfp) g_phase(s, c.te, B0);
+        const float2 v = s.fp[0];
+        if (store) {
+            out[n] = v;
+            nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+        if (PROG == kEpgSsfp) g_phase(s, c.tmte, B0);
+        if (PROG != kEpgSr) g_dephase<M, MULTI>(s, a, j);
+        g_relax(s, c.tmte, T1, T2, j);
+    }
+    if (store && a.norm2) a.norm2[(size_t)slice * a.natoms + atom] = nrm;
+}
+
+// ------------------------------------------------------------------------
+// ROA (rotation operators, src/functions.cu:1022-1222) with isochromats
+// ------------------------------------------------------------------------
+struct RoaArgs {
+    const TrCoef *coef;
+    const float *params;
+    float2 *atoms;
+    double *norm2;
+    size_t natoms;
+    int T;
+    int n_iso;
+    float spread;
+};
+
+template <int NI>
+__global__ __launch_bounds__(kBlock) void roa_kernel(RoaArgs a)
+{
+    const size_t atom = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (atom >= a.natoms) return;
+    const float T1 = a.params[atom * 4 + 0];
+    const float T2 = a.params[atom * 4 + 1];
+    const float B0 = a.params[atom * 4 + 2];
+    const float rT1 = 1.0f / T1, rT2 = 1.0f / T2;
+    float2 *out = a.atoms + atom * (size_t)a.T;
+    const float inv_n = 1.0f / (float)a.n_iso;
+    double nrm = 0.0;
+    // initial magnetisation of init_rf_pulse(180, 0, TI = 0) (:1022-1047)
+    float sa0, ca0;
+    sincosf((float)(180.0 * M_PI / 180.0), &sa0, &ca0);
+    for (int q0 = 0; q0 < a.n_iso; q0 += NI) {
+        float mx[NI], my[NI], mz[NI], b0q[NI];
+#pragma unroll
+        for (int u = 0; u < NI; u++) {
+            const int q = q0 + u;
+            const double off = (a.n_iso == 1)
+                ? 0.0 : (double)a.spread * (((double)q + 0.5) / (double)a.n_iso - 0.5);
+            b0q[u] = (float)((double)B0 + off);
+            mx[u] = 0.f;
+            my[u] = sa0;
+            mz[u] = ca0;
+        }
+        const bool last = q0 + NI >= a.n_iso;
+        for (int n = 0; n < a.T; n++) {
+            const TrCoef c = a.coef[n];
+            const float e1 = expf(-c.tr * rT1), e2 = expf(-c.tr * rT2);
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int u = 0; u < NI; u++) {
+                if (q0 + u < a.n_iso) {
+                    // fill_transition_matrix + Sgemm (:1049-1092)
+                    const float nx = c.c2p * mx[u] + c.s2p * c.ca * my[u] + c.s2p * c.sa * mz[u];
+                    const float ny = -c.s2p * mx[u] + c.c2p * c.ca * my[u] + c.c2p * c.sa * mz[u];
+                    const float nz = -c.sa * my[u] + c.ca * mz[u];
+                    // decay over TR (:1094-1105)
+                    float x = e2 * nx, y = e2 * ny;
+                    mz[u] = e1 * nz + (1.f - e1);
+                    // half-TR precession (:1107-1143); beta is a float
+                    const float beta = (float)(M_PI * (double)b0q[u] * (double)c.tr / 1000.0);
+                    float sb, cb;
+                    sincosf(beta, &sb, &cb);
+                    float x2 = cb * x + sb * y, y2 = cb * y - sb * x;
+                    sx += x2;
+                    sy += y2;
+                    mx[u] = cb * x2 + sb * y2;
+                    my[u] = cb * y2 - sb * x2;
+                }
+            }
+            float2 v = make_float2(sx, sy);
+            if (q0 > 0) {
+                const float2 o = out[n];
+                v = make_float2(o.x + sx, o.y + sy);
+            }
+            if (last && a.n_iso > 1) v = make_float2(v.x * inv_n, v.y * inv_n);
+            out[n] = v;
+            if (last) nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+    }
+    if (a.norm2) a.norm2[atom] = nrm;
+}
+
+__global__ __launch_bounds__(kBlock) void normalize_kernel(float2 *atoms, const double *norm2,
+                                                            size_t n, int T)
+{
+    const size_t row = blockIdx.x;
+    if (row >= n) return;
+    const double s = norm2[row];
+    const float f = s > 0.0 ? (float)(1.0 / sqrt(s)) : 1.0f;
+    float2 *p = atoms + row * (size_t)T;
+    for (int t = threadIdx.x; t < T; t += kBlock) {
+        const float2 v = p[t];
+        p[t] = make_float2(v.x * f, v.y * f);
+    }
+}
+
+// ------------------------------------------------------------------------
+// dispatch
+// ------------------------------------------------------------------------
+
+// Compiled state counts per lane.
+constexpr int kMs[] = {4, 8, 12, 16, 20, 24, 28, 30, 32};
+
+template <int M>
+int launch_m(const EpgLaunch &l, const EpgArgs &args, dim3 grid, hipStream_t s)
+{
+    const bool multi = args.L > 1;
+    const bool phase = l.echo == kEpgSsfp;
+    if (!l.generic && (l.echo == kEpgSsfp || l.echo == kEpgFisp)) {
+        if (phase) {
+            if (multi) epg_fused_kernel<M, true, true, false><<<grid, kBlock, 0, s>>>(args);
+            else epg_fused_kernel<M, false, true, false><<<grid, kBlock, 0, s>>>(args);
+        } else if (l.h_real) {
+            if (multi) epg_fused_kernel<M, true, false, true><<<grid, kBlock, 0, s>>>(args);
+            else epg_fused_kernel<M, false, false, true><<<grid, kBlock, 0, s>>>(args);
+        } else {
+            if (multi) epg_fused_kernel<M, true, false, false><<<grid, kBlock, 0, s>>>(args);
+            else epg_fused_kernel<M, false, false, false><<<grid, kBlock, 0, s>>>(args);
+        }
+        return 0;
+    }
+    if (M != 8 && M != 32) return -1;   // reference-order path: two lane widths
+#define MRF_GENERIC(PROG)                                                              \
+    do {                                                                               \
+        if (multi) epg_generic_kernel<M, true, PROG><<<grid, kBlock, 0, s>>>(args);    \
+        else epg_generic_kernel<M, false, PROG><<<grid, kBlock, 0, s>>>(args);         \
+    } while (0)
+    switch (l.echo) {
+    case kEpgSsfp: MRF_GENERIC(kEpgSsfp); break;
+    case kEpgFisp: MRF_GENERIC(kEpgFisp); break;
+    case kEpgSr: MRF_GENERIC(kEpgSr); break;
+    case kEpgSe: MRF_GENERIC(kEpgSe); break;
+    case kEpgTse: MRF_GENERIC(kEpgTse); break;
+    default: return -1;
+    }
+#undef MRF_GENERIC
+    return 0;
+}
+
+}  // namespace
+
+int launch_epg(const EpgLaunch &l, hipStream_t s)
+{
+    if (l.w < 2) return mrf_fail(MRF_E_ARG, "nstates must be >= 2 (got %d)", l.w);
+    const bool fused = !l.generic && (l.echo == kEpgSsfp || l.echo == kEpgFisp);
+    // choose lanes per atom L and states per lane M
+    // (fused: smallest L, then smallest compiled M with L*M >= w; reference-
+    // order path: M = 8 up to w = 128, else M = 32)
+    int L = 1, M = 0;
+    if (fused) {
+        for (L = 1; L <= 16 && !M; L *= 2)
+            for (int m : kMs)
+                if (m * L >= l.w) { M = m; break; }
+        L /= 2;
+    } else {
+        const int mw = (l.w <= 128) ? 8 : 32;
+        for (L = 1; L <= 16 && mw * L < l.w; L *= 2) {}
+        if (L <= 16) M = mw;
+    }
+    if (!M || L > 16) return mrf_fail(MRF_E_UNSUPPORTED, "nstates %d exceeds 512", l.w);
+
+    EpgArgs args{};
+    args.coef = l.coef;
+    args.params = l.params;
+    args.atoms = l.atoms;
+    args.norm2 = l.norm2;
+    args.natoms = l.natoms;
+    args.T = l.T;
+    args.L = L;
+    args.lshift = 0;
+    while ((1 << args.lshift) < L) args.lshift++;
+    const int cap = L * M;
+    args.istar = (cap > l.w) ? (l.w - 1) % M : -1;
+    args.jstar = (l.w - 1) / M;
+    // initial pulse angles as init_rf_pulse casts them (src/functions.cu:465-466)
+    float a_deg = 0.f, p_deg = 0.f, TI = 0.f;
+    switch (l.echo) {
+    case kEpgSsfp: a_deg = 180.f; p_deg = 0.f; TI = 40.f; break;   // :965-966, :1242
+    case kEpgSe:
+    case kEpgTse: a_deg = 90.f; p_deg = 90.f; break;                // :764-765, :833-834
+    default: break;                                                 // sr/fisp: (0, 0)
+    }
+    args.init_alpha = (float)((double)a_deg * M_PI / 180.0);
+    args.init_phi = (float)((double)p_deg * M_PI / 180.0);
+    args.TI = TI;
+
+    const size_t lanes = l.natoms * (size_t)L;
+    const size_t blocks = (lanes + kBlock - 1) / kBlock;
+    if (blocks > 0x7fffffff) return mrf_fail(MRF_E_ARG, "too many atoms");
+    dim3 grid((unsigned)blocks, (unsigned)l.n_slices);
+    int rc = -1;
+    switch (M) {
+    case 4: rc = launch_m<4>(l, args, grid, s); break;
+    case 8: rc = launch_m<8>(l, args, grid, s); break;
+    case 12: rc = launch_m<12>(l, args, grid, s); break;
+    case 16: rc = launch_m<16>(l, args, grid, s); break;
+    case 20: rc = launch_m<20>(l, args, grid, s); break;
+    case 24: rc = launch_m<24>(l, args, grid, s); break;
+    case 28: rc = launch_m<28>(l, args, grid, s); break;
+    case 30: rc = launch_m<30>(l, args, grid, s); break;
+    case 32: rc = launch_m<32>(l, args, grid, s); break;
+    default: break;
+    }
+    if (rc) return mrf_fail(MRF_E_UNSUPPORTED, "no EPG kernel for w=%d", l.w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "EPG launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int launch_roa(const RoaLaunch &l, hipStream_t s)
+{
+    RoaArgs a{l.coef, l.params, l.atoms, l.norm2, l.natoms, l.T, l.n_iso < 1 ? 1 : l.n_iso,
+              l.iso_spread_hz};
+    const size_t blocks = (l.natoms + kBlock - 1) / kBlock;
+    if (a.n_iso == 1) roa_kernel<1><<<dim3((unsigned)blocks), kBlock, 0, s>>>(a);
+    else roa_kernel<10><<<dim3((unsigned)blocks), kBlock, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "ROA launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int launch_normalize(float2 *atoms, const double *norm2, size_t n, int T, hipStream_t s)
+{
+    if (n == 0) return 0;
+    if (n > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many atoms to normalise");
+    normalize_kernel<<<dim3((unsigned)n), kBlock, 0, s>>>(atoms, norm2, n, T);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "normalize launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace mrf

@@ -1,0 +1,77 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (implemented in the
+// .hip translation units), called by the C-ABI layer in capi.hip.
+#ifndef SNAPMRF_KERNELS_H
+#define SNAPMRF_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "internal.h"
+
+namespace mrf {
+
+struct EpgLaunch {
+    const TrCoef *coef;    // device [n_slices][T]
+    const float *params;   // device [A][4]
+    float2 *atoms;         // device [n_slices * natoms][T]
+    double *norm2;         // device [n_slices * natoms] or nullptr
+    size_t natoms;         // atoms per slice
+    int n_slices;
+    int T;
+    int w;                 // EPG states
+    int echo;              // Echo
+    bool generic;          // force the unfused reference-order path
+    bool h_real;           // constant RF phase and no B0 rotation (FISP)
+};
+
+// EPG dictionary: SSFP/FISP fused path or SR/SE/TSE reference-order path.
+int launch_epg(const EpgLaunch &a, hipStream_t s);
+
+struct RoaLaunch {
+    const TrCoef *coef;    // device [T] (unscaled FA: the reference ignores B1)
+    const float *params;   // device [natoms][4]
+    float2 *atoms;         // device [natoms][T]
+    double *norm2;         // device [natoms] or nullptr
+    size_t natoms;
+    int T;
+    int n_iso;
+    float iso_spread_hz;
+};
+int launch_roa(const RoaLaunch &a, hipStream_t s);
+
+// atoms[a][t] *= 1/sqrt(norm2[a]) for norm2 > 0 (functions.cu:670-685).
+int launch_normalize(float2 *atoms, const double *norm2, size_t n, int T, hipStream_t s);
+
+// Matching ---------------------------------------------------------------
+
+struct MatchLaunch {
+    const float2 *atoms;        // [NATOMS][T]
+    const uint32_t *lowp_hi;    // [NATOMS][T] bf16x2 or nullptr
+    const uint32_t *lowp_lo;    // [NATOMS][T] bf16x2 or nullptr
+    const float *params;        // [NATOMS][4]
+    const float2 *sig;          // [V][T]
+    const int32_t *vox_slice;   // [V] slice index per voxel, or nullptr = all slices
+    size_t natoms;              // atoms per slice
+    int n_slices;
+    int T;
+    size_t V;
+    int mode;                   // MRF_MATCH_*
+    float *maps;                // [V][5]
+    int32_t *idx;               // [V] or nullptr
+};
+// Scratch size in bytes a matching call needs.
+size_t match_scratch_bytes(const MatchLaunch &a);
+int launch_match(const MatchLaunch &a, void *scratch, hipStream_t s);
+
+int launch_lowp_prepare(const float2 *atoms, size_t n, int T, uint32_t *hi, uint32_t *lo,
+                        hipStream_t s);
+
+// B1 slice per voxel: argmin_k |b1map[v] - slice_b1[k]| (strict <, init 10).
+int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,
+                     size_t V, int32_t *vox_slice, hipStream_t s);
+
+}  // namespace mrf
+
+#endif

@@ -1,0 +1,411 @@
+// match.hip -- fused complex-GEMM + argmax matcher for gfx950.
+//
+// S[atom][voxel] = sum_t conj(a_t) s_t is computed as two real GEMMs over the
+// interleaved (re, im) index k:
+//     Re S = sum_k A[k] X[k],   Im S = sum_k A[k] Y[k],
+//     X = (xr0, xi0, xr1, xi1, ...),  Y = (xi0, -xr0, xi1, -xr1, ...)
+// on v_mfma_f32_16x16x4f32 with rows = atoms, columns = voxels.  Each lane
+// reads 4 consecutive k of its row/column with one ds_read_b128 and uses
+// component s in MFMA step s; Y comes from the same 4 floats in registers
+// (x1, -x0, x3, -x2), so the signal is staged once.
+//
+// The score matrix is never written: after each 128-atom sub-tile every lane
+// folds its |S|^2 values into a running (score, index) per voxel, first index
+// winning ties as in generate_maps (src/functions.cu:1317-1349).  Work-groups
+// that share a voxel tile but cover different atom chunks merge through a
+// 64-bit atomicMax on (score bits << 32 | ~index), valid because scores >= 0.
+//
+// B1-map mode matches each voxel only against the slice its B1 map selects
+// (merge_maps(b1map), src/functions.cu:1351-1384): voxels are bucketed by
+// slice with a counting sort and tiled per slice.  Without a B1 map all NATOMS
+// atoms form one slice, which equals per-slice argmax + merge_maps(PD)
+// (src/functions.cu:1386-1416): max PD, ties to the lowest slice and index.
+#include "kernels.h"
+#include "snapmrf.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mrf {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int VT = 128;          // voxels per work-group tile
+constexpr int AT = 128;          // atoms per sub-tile
+constexpr int KC = 16;           // timepoints per K chunk
+constexpr int KF = 2 * KC;       // floats per row per chunk
+constexpr int LS = KF + 4;       // LDS row stride (floats), padded for ds_read_b128
+constexpr int kAtomChunk = 8192; // atoms per work-group (grid.y splits the slice)
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct Tile {
+    int slice;
+    int pstart;   // first position in the permuted voxel list
+    int count;    // voxels in this tile (<= VT)
+    int pad;
+};
+
+struct MatchArgs {
+    const float2 *atoms;
+    const float2 *sig;
+    const int32_t *perm;        // permuted voxel ids
+    const Tile *tiles;
+    const int *ntiles;
+    unsigned long long *keys;   // [V]
+    size_t natoms;              // atoms per slice
+    int T;
+};
+
+__device__ __forceinline__ unsigned long long make_key(float score, uint32_t idx)
+{
+    return ((unsigned long long)__float_as_uint(score) << 32) | (unsigned long long)(~idx);
+}
+
+__global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
+{
+    const int tile_id = blockIdx.x;
+    if (tile_id >= *a.ntiles) return;
+    const Tile tile = a.tiles[tile_id];
+    const size_t a_begin = (size_t)blockIdx.y * kAtomChunk;
+    if (a_begin >= a.natoms) return;
+    const size_t a_end = (a_begin + kAtomChunk < a.natoms) ? a_begin + kAtomChunk : a.natoms;
+    const size_t slice_base = (size_t)tile.slice * a.natoms;
+
+    __shared__ __attribute__((aligned(16))) float lds_a[AT * LS];
+    __shared__ __attribute__((aligned(16))) float lds_b[VT * LS];
+    __shared__ float red_s[2][4][VT];
+    __shared__ uint32_t red_i[2][4][VT];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wa = wave & 1;          // atom half of the sub-tile (64 atoms)
+    const int wv = wave >> 1;         // voxel half of the tile (64 voxels)
+    const int g = lane >> 4;          // k group
+    const int r16 = lane & 15;
+
+    // staging map: 8 float4 per row-chunk (KF = 32 floats); 256 threads cover
+    // 32 rows per pass, 4 passes for 128 rows.
+    const int st_row = tid >> 3;      // 0..31
+    const int st_col = (tid & 7) * 4; // float offset within the chunk row
+
+    // voxel rows of this tile (gathered through the permutation)
+    const float2 *vrow[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int vr = st_row + 32 * p;
+        vrow[p] = (vr < tile.count) ? a.sig + (size_t)a.perm[tile.pstart + vr] * (size_t)a.T
+                                    : nullptr;
+    }
+
+    float best[4];
+    uint32_t bidx[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        best[u] = -1.0f;
+        bidx[u] = 0xffffffffu;
+    }
+
+    const int nchunks = (a.T + KC - 1) / KC;
+    for (size_t sub = a_begin; sub < a_end; sub += AT) {
+        const float2 *arow[4];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const size_t ar = sub + st_row + 32 * p;
+            arow[p] = (ar < a_end) ? a.atoms + (slice_base + ar) * (size_t)a.T : nullptr;
+        }
+        floatx4 acc_re[4][4], acc_im[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc_im[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+
+        for (int c = 0; c < nchunks; c++) {
+            // global -> registers (zero outside [0, T) and outside the tile)
+            const int tc = c * KC + (st_col >> 1);   // first timepoint of this float4
+            float4 ga[4], gb[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
+                if (arow[p]) {
+                    if (tc + 1 < a.T) va = *reinterpret_cast<const float4 *>(arow[p] + tc);
+                    else if (tc < a.T) { float2 t = arow[p][tc]; va = make_float4(t.x, t.y, 0.f, 0.f); }
+                }
+                if (vrow[p]) {
+                    if (tc + 1 < a.T) vb = *reinterpret_cast<const float4 *>(vrow[p] + tc);
+                    else if (tc < a.T) { float2 t = vrow[p][tc]; vb = make_float4(t.x, t.y, 0.f, 0.f); }
+                }
+                ga[p] = va;
+                gb[p] = vb;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                *reinterpret_cast<float4 *>(&lds_a[(st_row + 32 * p) * LS + st_col]) = ga[p];
+                *reinterpret_cast<float4 *>(&lds_b[(st_row + 32 * p) * LS + st_col]) = gb[p];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < KF / 16; q++) {
+                floatx4 fa[4], fx[4], fy[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    fa[i] = *reinterpret_cast<const floatx4 *>(
+                        &lds_a[(wa * 64 + i * 16 + r16) * LS + q * 16 + g * 4]);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    fx[v] = *reinterpret_cast<const floatx4 *>(
+                        &lds_b[(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
+                    fy[v] = floatx4{fx[v][1], -fx[v][0], fx[v][3], -fx[v][2]};
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            acc_re[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                fa[i][s], fx[v][s], acc_re[i][v], 0, 0, 0);
+                            acc_im[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                fa[i][s], fy[v][s], acc_im[i][v], 0, 0, 0);
+                        }
+            }
+        }
+        // epilogue: D[row = 4g + r][col = r16] of each 16x16 tile
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const size_t atom = sub + wa * 64 + i * 16 + 4 * g + r;
+                if (atom < a_end) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const float re = acc_re[i][v][r], im = acc_im[i][v][r];
+                        const float sc = re * re + im * im;
+                        if (sc > best[v]) {
+                            best[v] = sc;
+                            bidx[v] = (uint32_t)(slice_base + atom);
+                        }
+                    }
+                }
+            }
+    }
+
+    // reduce the 8 candidates per voxel (4 k-groups x 2 atom halves)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        red_s[wa][g][wv * 64 + v * 16 + r16] = best[v];
+        red_i[wa][g][wv * 64 + v * 16 + r16] = bidx[v];
+    }
+    __syncthreads();
+    if (tid < VT && tid < tile.count) {
+        float bs = -1.0f;
+        uint32_t bi = 0xffffffffu;
+        for (int h = 0; h < 2; h++)
+            for (int q = 0; q < 4; q++) {
+                const float s2 = red_s[h][q][tid];
+                const uint32_t i2 = red_i[h][q][tid];
+                if (s2 > bs || (s2 == bs && i2 < bi)) {
+                    bs = s2;
+                    bi = i2;
+                }
+            }
+        if (bs >= 0.0f) {
+            const int vox = a.perm[tile.pstart + tid];
+            atomicMax(&a.keys[vox], make_key(bs, bi));
+        }
+    }
+}
+
+// --- voxel bucketing by B1 slice ------------------------------------------
+
+__global__ void b1_select_kernel(const float *b1map, const float *params, size_t natoms,
+                                 int n_slices, size_t V, int32_t *vox_slice)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    // merge_maps(b1map): strict '<' from 10, first slice on ties, float math
+    const float b = b1map[v];
+    int m = 0;
+    float bd = 10.0f;
+    for (int k = 0; k < n_slices; k++) {
+        const float d = fabsf(b - params[(size_t)k * natoms * 4 + 3]);
+        if (d < bd) {
+            bd = d;
+            m = k;
+        }
+    }
+    vox_slice[v] = m;
+}
+
+__global__ void count_kernel(const int32_t *vox_slice, size_t V, int *counts)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    atomicAdd(&counts[vox_slice ? vox_slice[v] : 0], 1);
+}
+
+// single work-item: exclusive scan of the slice counts and the tile list
+__global__ void tiles_kernel(const int *counts, int n_slices, int *offsets, Tile *tiles,
+                             int *ntiles)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int off = 0, nt = 0;
+    for (int k = 0; k < n_slices; k++) {
+        offsets[k] = off;
+        const int c = counts[k];
+        for (int p = 0; p < c; p += VT) {
+            Tile t;
+            t.slice = k;
+            t.pstart = off + p;
+            t.count = (c - p < VT) ? c - p : VT;
+            t.pad = 0;
+            tiles[nt++] = t;
+        }
+        off += c;
+    }
+    *ntiles = nt;
+}
+
+__global__ void scatter_kernel(const int32_t *vox_slice, size_t V, const int *offsets,
+                               int *fill, int32_t *perm)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int k = vox_slice ? vox_slice[v] : 0;
+    const int p = offsets[k] + atomicAdd(&fill[k], 1);
+    perm[p] = (int32_t)v;
+}
+
+__global__ void maps_kernel(const unsigned long long *keys, const float *params, size_t V,
+                            float *maps, int32_t *idx)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const unsigned long long k = keys[v];
+    uint32_t i = ~(uint32_t)(k & 0xffffffffull);
+    float sc = __uint_as_float((uint32_t)(k >> 32));
+    if (k == 0ull) {   // no candidate (empty slice): reference leaves index 0
+        i = 0;
+        sc = 0.0f;
+    }
+    const float *p = params + (size_t)i * 4;
+    maps[v * 5 + 0] = p[0];
+    maps[v * 5 + 1] = p[1];
+    maps[v * 5 + 2] = p[2];
+    maps[v * 5 + 3] = p[3];
+    maps[v * 5 + 4] = sqrtf(sc);
+    if (idx) idx[v] = (int32_t)i;
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void zero_i32_kernel(int *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Scratch {
+    unsigned long long *keys;
+    int32_t *perm;
+    int32_t *vox_slice_own;
+    int *counts, *offsets, *fill, *ntiles;
+    Tile *tiles;
+    size_t max_tiles;
+};
+
+Scratch carve(void *base, const MatchLaunch &l)
+{
+    Scratch s{};
+    const int ns = l.vox_slice ? l.n_slices : 1;
+    s.max_tiles = (l.V + VT - 1) / VT + (size_t)ns;
+    uint8_t *p = (uint8_t *)base;
+    s.keys = (unsigned long long *)p;
+    p += align256(l.V * 8);
+    s.perm = (int32_t *)p;
+    p += align256(l.V * 4);
+    s.counts = (int *)p;
+    p += align256((size_t)ns * 4);
+    s.offsets = (int *)p;
+    p += align256((size_t)ns * 4);
+    s.fill = (int *)p;
+    p += align256((size_t)ns * 4);
+    s.ntiles = (int *)p;
+    p += 256;
+    s.tiles = (Tile *)p;
+    return s;
+}
+
+}  // namespace
+
+size_t match_scratch_bytes(const MatchLaunch &l)
+{
+    const int ns = l.vox_slice ? l.n_slices : 1;
+    const size_t max_tiles = (l.V + VT - 1) / VT + (size_t)ns;
+    return align256(l.V * 8) + align256(l.V * 4) + 3 * align256((size_t)ns * 4) + 256 +
+           align256(max_tiles * sizeof(Tile));
+}
+
+int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,
+                     size_t V, int32_t *vox_slice, hipStream_t s)
+{
+    if (V == 0) return 0;
+    b1_select_kernel<<<dim3((unsigned)((V + 255) / 256)), 256, 0, s>>>(b1map, params, natoms,
+                                                                        n_slices, V, vox_slice);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "b1 select: %s", hipGetErrorString(e));
+}
+
+int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
+{
+    if (l.V == 0) return 0;
+    if (l.mode != MRF_MATCH_FP32)
+        return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", l.mode);
+    const int ns = l.vox_slice ? l.n_slices : 1;
+    const size_t nat = l.vox_slice ? l.natoms : l.natoms * (size_t)l.n_slices;
+    if ((size_t)ns * nat > 0xfffffffeULL)
+        return mrf_fail(MRF_E_ARG, "dictionary larger than 2^32-1 atoms");
+    if (l.V > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many voxels");
+    Scratch sc = carve(scratch, l);
+    const unsigned vb = (unsigned)((l.V + 255) / 256);
+    zero_u64_kernel<<<vb, 256, 0, s>>>(sc.keys, l.V);
+    zero_i32_kernel<<<1, 256, 0, s>>>(sc.counts, (size_t)ns);
+    zero_i32_kernel<<<1, 256, 0, s>>>(sc.fill, (size_t)ns);
+    if (ns > 256) {
+        zero_i32_kernel<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(sc.counts, (size_t)ns);
+        zero_i32_kernel<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(sc.fill, (size_t)ns);
+    }
+    count_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, sc.counts);
+    tiles_kernel<<<1, 1, 0, s>>>(sc.counts, ns, sc.offsets, sc.tiles, sc.ntiles);
+    scatter_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, sc.offsets, sc.fill, sc.perm);
+
+    MatchArgs a{};
+    a.atoms = l.atoms;
+    a.sig = l.sig;
+    a.perm = sc.perm;
+    a.tiles = sc.tiles;
+    a.ntiles = sc.ntiles;
+    a.keys = sc.keys;
+    a.natoms = nat;
+    a.T = l.T;
+    const size_t chunks = (nat + kAtomChunk - 1) / kAtomChunk;
+    dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
+    match_fp32_kernel<<<grid, kBlock, 0, s>>>(a);
+    maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
+}
+
+}  // namespace mrf

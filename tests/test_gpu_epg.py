@@ -1,0 +1,140 @@
+"""GPU parity of the dictionary kernels (through the C ABI) against the
+reference fixtures and the FP64 oracle.  Tolerance: 1e-5 of each atom's
+infinity norm (the reference's own fixtures carry FP32 noise of ~5e-6)."""
+import numpy as np
+import pytest
+
+import oracle
+import ra_util
+import snapmrf_amd as sm
+from snapmrf_amd import synth
+from helpers import const_seq, fixture_params, mrf5_raw, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+SIMPLE = {
+    "epg_se": ((np.pi, 0.0, 1000.0, 25.0, 1), (600, 100)),
+    "epg_sr": ((np.pi / 3, np.pi / 2, 500.0, 1.0, 10), (600, 100)),
+    "epg_tse1": ((2 * np.pi / 3, 0.0, 0.0, 0.0, 3), (1000, 100)),
+    "epg_tse2": ((np.pi, 0.0, 50.0, 25.0, 10), (600, 100)),
+    "epg_fisp": ((np.pi / 6, 0.0, 10.0, 5.0, 3), (1000, 100)),
+}
+
+
+def gen(torch, echo, raw, params, natoms, w, T=None, **kw):
+    """Runs the dictionary on the GPU; returns complex128 numpy [A, T]."""
+    T = raw.shape[1] if T is None else T
+    A = params.shape[0]
+    d_params = torch.from_numpy(np.ascontiguousarray(params)).cuda()
+    d_atoms = torch.zeros((A, T), dtype=torch.complex64, device="cuda")
+    if kw:
+        sm.dict_ex(echo, raw, T, d_params, A, natoms, w, d_atoms, **kw)
+        torch.cuda.synchronize()
+    else:
+        sm.MRF_dict(d_atoms, d_params, raw, T, raw.shape[1], A, natoms, 4, w, echo)
+    return d_atoms.cpu().numpy().astype(np.complex128)
+
+
+@pytest.mark.parametrize("echo", sorted(SIMPLE))
+def test_reference_unit_tests(gpu, golden, echo):
+    """src/test.cu:43-156 with nstates = 101 (4 lanes per atom)."""
+    (a, p, tr, te, T), (t1, t2) = SIMPLE[echo]
+    raw = const_seq(T, a, p, tr, te)
+    params = np.array([[t1, t2, 0, 1]], np.float32)
+    got = gen(gpu, echo, raw, params, 1, 101)
+    gt = ra_util.read(f"{golden}/atoms_{echo}.ra")
+    assert np.abs(got - gt).max() < 1e-6
+    ref = oracle.epg(echo, oracle.seq_prepare(echo, raw), params, 1, 101)
+    assert rel_err(got, ref) < TOL
+
+
+@pytest.mark.parametrize("echo", ["epg_ssfp", "roa_ssfp"])
+def test_reference_ssfp_unit_tests(gpu, golden, echo):
+    raw = mrf5_raw(golden)
+    params = fixture_params()
+    got = gen(gpu, echo, raw, params, 4, 101)
+    gt = ra_util.read(f"{golden}/atoms_{echo}.ra")
+    assert rel_err(got, gt) < TOL
+    seq = oracle.seq_prepare(echo, raw)
+    ref = oracle.epg(echo, seq, params, 4, 101) if echo == "epg_ssfp" else oracle.roa(seq, params, 4)
+    assert rel_err(got, ref) < TOL
+
+
+def test_fisp_config1_vs_oracle(gpu):
+    """FISP, T1 100:20:2000, T2 20:10:300 (2,663 atoms), T = 1000, w = 20."""
+    s = synth.sequence(1000, seed=1)
+    raw = synth.csv_columns(s, "epg_fisp")
+    params, A, natoms = sm.build_params("100:20:2000", "20:10:300", "0", "1")
+    got = gen(gpu, "epg_fisp", raw, params, natoms, 20)
+    sub = np.arange(0, A, 7)
+    ref = oracle.epg("epg_fisp", oracle.seq_prepare("epg_fisp", raw), params[sub], len(sub), 20)
+    assert rel_err(got[sub], ref) < TOL
+
+
+@pytest.mark.parametrize("w", [30, 60, 101, 200])
+def test_ssfp_b0_b1_grid_vs_oracle(gpu, w):
+    """B1-sliced SSFP with off-resonance; w = 30 (1 lane/atom), 60 (2),
+    101 (4, padded), 200 (8, padded)."""
+    T = 400 if w > 60 else 1000
+    s = synth.sequence(T, seed=2)
+    raw = synth.csv_columns(s, "epg_ssfp")
+    params, A, natoms = sm.build_params("100:300:2000+2300:900:5000", "20:40:300+400:800:2000",
+                                        "-40:20:40", "0.8:0.2:1.2")
+    got = gen(gpu, "epg_ssfp", raw, params, natoms, w)
+    ref = oracle.epg("epg_ssfp", oracle.seq_prepare("epg_ssfp", raw), params, natoms, w)
+    assert rel_err(got, ref) < TOL
+    assert np.allclose(np.linalg.norm(got, axis=1), 1.0, atol=1e-5)
+
+
+def test_fused_and_reference_order_paths_agree(gpu):
+    s = synth.sequence(300, seed=4)
+    s["phi_deg"] = np.linspace(0, 90, 300)     # varying RF phase exercises H's phase ratio
+    for echo in ("epg_ssfp", "epg_fisp"):
+        raw = synth.csv_columns(s, echo)
+        params, A, natoms = sm.build_params("200:400:2000", "30:60:300", "-25:25:25", "0.9:0.2:1.1")
+        if echo == "epg_fisp":
+            params, A, natoms = params[:natoms], natoms, natoms
+        fused = gen(gpu, echo, raw, params, natoms, 24, normalize=0)
+        plain = gen(gpu, echo, raw, params, natoms, 24, normalize=0, generic=True)
+        assert rel_err(fused, plain) < TOL
+        ref = oracle.epg(echo, oracle.seq_prepare(echo, raw), params, natoms, 24)
+        if echo == "epg_ssfp":
+            plain = plain / np.linalg.norm(plain, axis=1, keepdims=True)
+        assert rel_err(plain, ref) < TOL
+
+
+def test_roa_isochromats_vs_oracle(gpu):
+    s = synth.sequence(500, seed=5)
+    raw = synth.csv_columns(s, "roa_ssfp")
+    params, A, natoms = sm.build_params("200:300:2000", "30:50:300", "-30:15:30", "1")
+    seq = oracle.seq_prepare("roa_ssfp", raw)
+    for n_iso, spread in ((1, 0.0), (100, 60.0), (13, 25.0)):
+        got = gen(gpu, "roa_ssfp", raw, params, natoms, 2, n_iso=n_iso, iso_spread_hz=spread)
+        ref = oracle.roa(seq, params, natoms, n_iso=n_iso, spread_hz=spread)
+        assert rel_err(got, ref) < TOL, (n_iso, spread)
+
+
+def test_ssfp_b0_conjugate_symmetry_full_size(gpu):
+    """Size-independent property at T = 1000, w = 30 on the full T1/T2 axes of
+    the 871,455-atom grid (one B1 slice, B0 = +-40 Hz)."""
+    s = synth.sequence(1000, seed=0)
+    raw = synth.csv_columns(s, "epg_ssfp")
+    params, A, natoms = sm.build_params("100:20:2000+2300:300:5000",
+                                        "20:5:100+110:10:300+400:200:2000", "-40:80:40", "1")
+    got = gen(gpu, "epg_ssfp", raw, params, natoms, 30)
+    half = A // 2
+    assert np.abs(got[:half] + np.conj(got[half:])).max() < 1e-6
+    assert np.allclose(np.linalg.norm(got, axis=1), 1.0, atol=1e-5)
+
+
+def test_bad_arguments_raise(gpu):
+    d_params = gpu.zeros((4, 4), device="cuda")
+    d_atoms = gpu.zeros((4, 5), dtype=gpu.complex64, device="cuda")
+    raw = np.zeros((4, 5), np.float32)
+    with pytest.raises(sm.MRFError):
+        sm.MRF_dict(d_atoms, d_params, raw, 5, 5, 4, 4, 4, 101, "epg_nope")
+    with pytest.raises(sm.MRFError):
+        sm.MRF_dict(d_atoms, d_params, raw, 5, 5, 4, 3, 4, 101, "epg_ssfp")   # NATOMS % natoms
+    with pytest.raises(sm.MRFError):
+        sm.MRF_dict(d_atoms, d_params, raw, 5, 5, 4, 4, 4, 1, "epg_ssfp")     # w < 2

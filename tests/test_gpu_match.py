@@ -1,0 +1,153 @@
+"""GPU parity of the fused matcher (through the C ABI) against the
+reference's matching KAT and the FP64 oracle.
+
+The reference FP32 matcher is cuBLAS CGEMM whose summation order is not
+reproducible, so FP32 argmax parity is stated against the FP64 oracle:
+indices must agree exactly on every voxel whose FP64 top-2 score gap exceeds
+the FP32 error bound (relative gap > 1e-5 here); PD agrees to 1e-5 relative."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import ra_util
+import snapmrf_amd as sm
+from snapmrf_amd import synth
+
+pytestmark = pytest.mark.gpu
+GAP = 1e-5
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def match(torch, atoms, params, sig, natoms, b1map=None, mode=sm.MRF_MATCH_FP32):
+    """atoms complex64 [A, T] (numpy), signals complex64 [V, T] -> (maps [V, 5], idx [V])."""
+    A, T = atoms.shape
+    V = sig.shape[0]
+    d_atoms, d_params, d_sig = dev(torch, atoms), dev(torch, params), dev(torch, sig)
+    d_maps = torch.zeros((V, 5), device="cuda")
+    d_idx = torch.zeros(V, dtype=torch.int32, device="cuda")
+    d_b1 = None if b1map is None else dev(torch, b1map.astype(np.float32))
+    sm.match_ex(mode, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps, d_b1map=d_b1, d_idx=d_idx)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+def test_reference_matching_kat(gpu, golden):
+    """unittest_matching (src/functions.cu:1720-1824) through the drop-in
+    MRF_match with a host image."""
+    atoms = ra_util.read(f"{golden}/atoms_mrf.ra")
+    params = ra_util.read(f"{golden}/params_mrf.ra").copy()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    idx = [libc.rand() % 109 for _ in range(10)]
+    img = atoms[idx].copy()
+    d_atoms, d_params = dev(gpu, atoms), dev(gpu, params)
+    d_maps = gpu.zeros((10, 5), device="cuda")
+    sm.MRF_match(d_maps, img, d_atoms, d_params, 100, 100, 109, 109, 10, 4, 5, None)
+    maps = d_maps.cpu().numpy()
+    assert np.abs(maps[:, :4] - params[idx]).sum() < 1e-6
+    assert np.allclose(maps[:, 4], 1.0, atol=1e-5)
+
+
+def _fisp_dict(torch, T=1000, seed=1):
+    raw = synth.csv_columns(synth.sequence(T, seed), "epg_fisp")
+    params, A, natoms = sm.build_params("100:20:2000", "20:10:300", "0", "1")
+    d_params = dev(torch, params)
+    d_atoms = torch.zeros((A, T), dtype=torch.complex64, device="cuda")
+    sm.dict_ex("epg_fisp", raw, T, d_params, A, natoms, 20, d_atoms, normalize=1)
+    torch.cuda.synchronize()
+    return d_atoms.cpu().numpy(), params, natoms
+
+
+def _check_against_oracle(maps, idx, atoms, params, sig, base=0):
+    ref_idx, ref_pd, ref_sec = oracle.match_slice(atoms, sig)
+    gap = (ref_pd - ref_sec) / np.maximum(ref_pd, 1e-30)
+    clear = gap > GAP
+    assert clear.mean() > 0.5
+    assert np.array_equal(idx[clear], ref_idx[clear] + base)
+    assert np.allclose(maps[:, 4], ref_pd, rtol=1e-5, atol=1e-7)
+    assert np.array_equal(maps[clear, :4], params[ref_idx[clear] + base])
+    return int((~clear).sum())
+
+
+def test_config1_fp32_vs_fp64_oracle(gpu):
+    """2,663 FISP atoms, 4,096 voxels at SNR 30."""
+    atoms, params, natoms = _fisp_dict(gpu)
+    sig, truth, pd = synth.signals_np(atoms, 4096, seed=7)
+    maps, idx = match(gpu, atoms, params, sig, natoms)
+    _check_against_oracle(maps, idx, atoms, params, sig)
+
+
+@pytest.mark.parametrize("T,V", [(37, 1), (37, 129), (16, 300), (1000, 257)])
+def test_ragged_shapes(gpu, T, V):
+    rng = np.random.default_rng(T * 1000 + V)
+    A = 300
+    atoms = (rng.normal(size=(A, T)) + 1j * rng.normal(size=(A, T))).astype(np.complex64)
+    atoms /= np.linalg.norm(atoms, axis=1, keepdims=True)
+    params = np.stack([np.arange(A) + 1000, np.arange(A) % 17 + 1, np.zeros(A), np.ones(A)], 1)
+    params = params.astype(np.float32)
+    sig, truth, pd = synth.signals_np(atoms, V, seed=3, snr=5.0)
+    maps, idx = match(gpu, atoms, params, sig, A)
+    _check_against_oracle(maps, idx, atoms, params, sig)
+
+
+def test_ties_go_to_first_index(gpu):
+    rng = np.random.default_rng(11)
+    T, A = 64, 9000           # spans two atom chunks of the kernel
+    atoms = (rng.normal(size=(A, T)) + 1j * rng.normal(size=(A, T))).astype(np.complex64)
+    atoms[8500] = atoms[17]   # exact duplicates in different work-groups
+    atoms[8999] = atoms[4100]
+    atoms[130] = atoms[129]   # and inside one 128-atom sub-tile boundary
+    params = np.zeros((A, 4), np.float32)
+    params[:, 0] = np.arange(A)
+    sig = np.stack([atoms[17], atoms[4100], atoms[129], np.zeros(T, np.complex64),
+                    2.0 * atoms[8500]])
+    maps, idx = match(gpu, atoms, params, sig, A)
+    assert idx.tolist() == [17, 4100, 129, 0, 17]
+    assert maps[3, 4] == 0.0
+
+
+def _ssfp_dict(torch, T=300, seed=2):
+    raw = synth.csv_columns(synth.sequence(T, seed), "epg_ssfp")
+    params, A, natoms = sm.build_params("100:100:2000", "20:20:300", "-30:15:30", "0.8:0.1:1.2")
+    d_params = dev(torch, params)
+    d_atoms = torch.zeros((A, T), dtype=torch.complex64, device="cuda")
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, 30, d_atoms)
+    torch.cuda.synchronize()
+    return d_atoms.cpu().numpy(), params, natoms
+
+
+def test_b1_map_selection_vs_oracle(gpu):
+    """merge_maps(b1map) semantics: each voxel takes the best atom of the
+    slice whose B1 is nearest its map value (src/functions.cu:1351-1384)."""
+    atoms, params, natoms = _ssfp_dict(gpu)
+    l_b1 = atoms.shape[0] // natoms
+    b1 = synth.b1_map(32, 40, seed=3).ravel()
+    b1[:5] = [0.85, 0.95, 0.0, 7.0, 1.2]          # ties between slices, out-of-range values
+    slice_b1 = params[::natoms, 3]
+    ks = np.array([oracle.b1_slice(v, slice_b1) for v in b1])
+    # each voxel's signal comes from its own slice
+    rng = np.random.default_rng(5)
+    pick = ks * natoms + rng.integers(0, natoms, len(b1))
+    sig, _, _ = synth.signals_np(atoms, len(b1), seed=9, candidates=None)
+    sig = (atoms[pick] * 0.7 + (sig - sig.mean()) * 0.01).astype(np.complex64)
+    maps, idx = match(gpu, atoms, params, sig, natoms, b1map=b1)
+    for k in range(l_b1):
+        sel = ks == k
+        if not sel.any():
+            continue
+        sl = slice(k * natoms, (k + 1) * natoms)
+        _check_against_oracle(maps[sel], idx[sel], atoms[sl], params, sig[sel], base=k * natoms)
+
+
+def test_pd_merge_is_global_first_index_argmax(gpu):
+    """Without a B1 map: merge_maps(PD) over per-slice argmaxes
+    (src/functions.cu:1386-1416) = global argmax, ties to the lowest index."""
+    atoms, params, natoms = _ssfp_dict(gpu)
+    sig, truth, pd = synth.signals_np(atoms, 1000, seed=4, snr=20.0)
+    maps, idx = match(gpu, atoms, params, sig, natoms)
+    _check_against_oracle(maps, idx, atoms, params, sig)
