@@ -67,6 +67,61 @@ def cpu_baseline(seq_raw, params, natoms, T, w, budget_s=8.0):
                 match_value=vox_per_s, match_unit="voxels/s")
 
 
+def make_signals(torch, np, synth, d_atoms, params, natoms, T, nx=240, ny=240, snr=25.0):
+    """240x240 voxels: atom of the slice nearest the smooth B1 map value,
+    PD ~ U(0.3, 1), complex Gaussian noise at the given SNR (on the GPU)."""
+    V = nx * ny
+    b1 = synth.b1_map(nx, ny, seed=0).ravel()
+    slice_b1 = params[::natoms, 3]
+    d = np.abs(b1[:, None] - slice_b1[None, :])
+    ks = np.argmin(np.where(d < 10.0, d, 10.0), axis=1)
+    rng = np.random.default_rng(1)
+    truth = ks * natoms + rng.integers(0, natoms, V)
+    pd = rng.uniform(0.3, 1.0, V).astype(np.float32)
+    d_truth = torch.from_numpy(truth).cuda()
+    clean = d_atoms[d_truth] * torch.from_numpy(pd).cuda()[:, None]
+    rms = clean.abs().pow(2).mean(dim=1).sqrt()
+    g = torch.Generator(device="cuda").manual_seed(2)
+    noise = torch.complex(torch.randn(V, T, device="cuda", generator=g),
+                          torch.randn(V, T, device="cuda", generator=g))
+    d_sig = (clean + noise * (rms / (snr * 2 ** 0.5))[:, None]).to(torch.complex64).contiguous()
+    return d_sig, torch.from_numpy(b1).cuda(), truth, V
+
+
+def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_atoms, d_norm2,
+                stream):
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms, d_norm2=d_norm2,
+               normalize=0, stream=stream)
+    torch.cuda.synchronize()
+    log("profile: EPG launch done")
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms, d_norm2=d_norm2,
+               stream=stream)
+    d_sig, d_b1, truth, V = make_signals(torch, np, synth, d_atoms, params, natoms, T)
+    d_maps = torch.empty((V, 5), device="cuda")
+    sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps,
+                d_b1map=d_b1, stream=stream)
+    torch.cuda.synchronize()
+    log("profile: matching done")
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("profile: 1 GiB calibration copy done")
+    return 0
+
+
+def measured_traffic(key: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (profiles/traffic.json, FETCH_SIZE + WRITE_SIZE scaled by the
+    1 GiB copy calibration), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)[key]
+        return t["fetch_bytes"] + t["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +130,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--small", action="store_true", help="reduced grid (debugging only)")
     ap.add_argument("--skip-match", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="one EPG launch, one matching call and a 1 GiB device copy (the "
+                         "HBM-counter calibration), for rocprofv3 runs; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole path on a single "
@@ -116,6 +174,10 @@ def main() -> int:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
+    if args.profile:
+        return profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params,
+                           d_atoms, d_norm2, stream)
+
     for _ in range(args.warmup):
         gen()
     torch.cuda.synchronize()
@@ -141,30 +203,14 @@ def main() -> int:
                    "atoms": A, "b1_slices": l_b1, "T": T, "w": w, "voxels": 240 * 240,
                    "grid": {"T1": grid[0], "T2": grid[1], "B0": grid[2], "B1": grid[3]}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
+                     "frac": achieved / PEAK_FP32, "traffic": measured_traffic("epg"),
                      "kernel": "epg_fused_kernel<30,false,true,false>",
                      "algorithmic_flops_per_launch": flops, "ms_per_launch": kern_ms},
         "ref_published_atoms_per_s": 10765,
     }
 
     if not args.skip_match:
-        V = 240 * 240
-        b1 = synth.b1_map(240, 240, seed=0).ravel()
-        slice_b1 = params[::natoms, 3]
-        ks = np.argmin(np.where(np.abs(b1[:, None] - slice_b1[None, :]) < 10.0,
-                                np.abs(b1[:, None] - slice_b1[None, :]), 10.0), axis=1)
-        rng = np.random.default_rng(1)
-        truth = ks * natoms + rng.integers(0, natoms, V)
-        pd = rng.uniform(0.3, 1.0, V).astype(np.float32)
-        d_truth = torch.from_numpy(truth).cuda()
-        clean = d_atoms[d_truth] * torch.from_numpy(pd).cuda()[:, None]
-        rms = clean.abs().pow(2).mean(dim=1).sqrt()
-        gen_t = torch.Generator(device="cuda").manual_seed(2)
-        noise = torch.complex(torch.randn(V, T, device="cuda", generator=gen_t),
-                              torch.randn(V, T, device="cuda", generator=gen_t))
-        d_sig = (clean + noise * (rms / (25.0 * 2 ** 0.5))[:, None]).to(torch.complex64).contiguous()
-        del clean, noise
-        d_b1 = torch.from_numpy(b1).cuda()
+        d_sig, d_b1, truth, V = make_signals(torch, np, synth, d_atoms, params, natoms, T)
         d_maps = torch.empty((V, 5), device="cuda")
         d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
 
@@ -186,7 +232,8 @@ def main() -> int:
         out["match_steps"] = msteps
         out["match_truth_agree"] = float(np.mean(idx == truth))
         out["roofline_match"] = {"bound": "mfma", "achieved": ach_m, "peak": PEAK_FP32,
-                                 "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32, "traffic": None,
+                                 "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32,
+                                 "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
                                  "algorithmic_flops_per_launch": mflops}
     if not args.no_cpu:

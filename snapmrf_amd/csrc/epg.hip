@@ -88,6 +88,40 @@ __device__ __forceinline__ float phase_angle(float B0, float t)
     return (float)(2.0 * M_PI * (double)B0 * (double)t / 1000.0);
 }
 
+// sin and cos of 2*pi*r for r in revolutions.  The range reduction runs in
+// double (r - rint(r), quadrant split, scaling by 2*pi), so the float
+// argument of the polynomials carries only its own final rounding; the
+// polynomials are the Cephes single-precision minimax fits on [-pi/4, pi/4].
+__device__ __forceinline__ void sincos_rev(double r, float *s, float *c)
+{
+    r -= rint(r);                                   // [-1/2, 1/2]
+    const double q = rint(4.0 * r);                 // quadrant, -2..2
+    const float x = (float)((r - 0.25 * q) * 6.283185307179586);   // [-pi/4, pi/4]
+    const float x2 = x * x;
+    const float sp = x + x * x2 * fmaf(x2, fmaf(x2, -1.9515295891e-4f, 8.3321608736e-3f),
+                                       -1.6666654611e-1f);
+    const float cp = fmaf(x2 * x2, fmaf(x2, fmaf(x2, 2.443315711809948e-5f,
+                                                 -1.388731625493765e-3f),
+                                        4.166664568298827e-2f),
+                          fmaf(x2, -0.5f, 1.0f));
+    const int iq = (int)q & 3;
+    const float ss = (iq & 1) ? cp : sp;
+    const float cc = (iq & 1) ? sp : cp;
+    *s = (iq & 2) ? -ss : ss;
+    *c = ((iq + 1) & 2) ? -cc : cc;
+}
+
+// exp(-t/T) as 2^(t*k) with k = -log2(e)/T formed once per atom in double.
+__device__ __forceinline__ float decay_k(float T)
+{
+    return (float)(-1.4426950408889634 / (double)T);
+}
+
+__device__ __forceinline__ float decay(float t, float k)
+{
+    return __builtin_amdgcn_exp2f(t * k);
+}
+
 template <int M>
 struct EpgState {
     float2 fp[M], fm[M], z[M];
@@ -131,103 +165,188 @@ __device__ __forceinline__ void epg_truncate(EpgState<M> &s, const EpgArgs &a, i
 // ------------------------------------------------------------------------
 // Fused SSFP / FISP kernel
 // ------------------------------------------------------------------------
-template <int M, bool MULTI, bool PHASE, bool HREAL>
-__global__ __launch_bounds__(kBlock) void epg_fused_kernel(EpgArgs a)
+
+// TR loop unroll factor.  Within one TR the states are swept from the top
+// (k = M-1) down: new F+_{k+1} overwrites a register whose old value was
+// consumed one step earlier, and new F-_{k-1} overwrites the register of the
+// old F-_k read in the same step.  The F- array therefore drifts up by one
+// index per TR (slot k lives at fm[k + r] after r TRs) and is copied back
+// once per kUnroll TRs, so neither shift needs per-TR register moves.
+constexpr int kUnroll = 2;
+
+template <int M>
+struct FusedState {
+    float2 fp[M];
+    float2 fm[M + kUnroll];
+    float2 z[M];
+};
+
+struct FusedCtx {
+    float k1, k2;     // -log2(e)/T1, -log2(e)/T2
+    double b0r;       // B0 in revolutions per ms
+    int j;            // lane within the atom's group
+};
+
+// One TR of the fused frame-form step with the F- array offset R; returns
+// the sample E2(TE) e^{i theta_TE} e^{i phi_n} P+_0.
+template <int M, int R, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
+__device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
+                                             const FusedCtx &x, const EpgArgs &a)
+{
+    const float ea = decay(c.te, x.k2);
+    const float eb = decay(c.tmte, x.k2);
+    const float E1 = decay(c.tr, x.k1);
+    float2 H, csave;
+    if (PHASE) {
+        // B0 precession over TE and TR-TE (src/functions.cu:585-607)
+        float sa_, ca_, sb_, cb_;
+        sincos_rev(x.b0r * (double)c.te, &sa_, &ca_);
+        sincos_rev(x.b0r * (double)c.tmte, &sb_, &cb_);
+        const float2 e = cmul(make_float2(ca_, sa_), make_float2(cb_, sb_));
+        const float g = ea * eb;
+        const float2 er = cmul(e, make_float2(c.rr, c.ri));
+        H = make_float2(g * er.x, g * er.y);
+        const float2 ez = cmul(make_float2(ca_, sa_), make_float2(c.zr, c.zi));
+        csave = make_float2(ea * ez.x, ea * ez.y);
+    } else {
+        const float g = ea * eb;
+        H = HREAL ? make_float2(g, 0.f) : make_float2(g * c.rr, g * c.ri);
+        csave = make_float2(ea * c.zr, ea * c.zi);
+    }
+    const float ca2 = c.ca2, sa2 = c.sa2, sa = c.sa;
+    const float eca = E1 * c.ca, eh = E1 * 0.5f * c.sa;
+
+    float2 top = make_float2(0.f, 0.f), d0 = top, fp0 = top, p0 = top;
+#pragma unroll
+    for (int i = M - 1; i >= 0; i--) {
+        const float2 A = s.fp[i], B = s.fm[i + R], C = s.z[i];
+        const float2 pp = make_float2(ca2 * A.x + sa2 * B.x + sa * C.y,
+                                      ca2 * A.y + sa2 * B.y - sa * C.x);
+        const float2 pm = make_float2(sa2 * A.x + ca2 * B.x - sa * C.y,
+                                      sa2 * A.y + ca2 * B.y + sa * C.x);
+        const float dx = A.x - B.x, dy = A.y - B.y;
+        s.z[i] = make_float2(eca * C.x + eh * dy, eca * C.y - eh * dx);
+        const float2 hp = HREAL ? make_float2(H.x * pp.x, H.x * pp.y) : cmul(H, pp);
+        const float2 hm = HREAL ? make_float2(H.x * pm.x, H.x * pm.y) : cmulc(H, pm);
+        if (i == M - 1) top = hp;
+        else s.fp[i + 1] = hp;            // new F+_{i+1}
+        if (i >= 1) s.fm[i + R] = hm;     // new F-_{i-1}, at index (i-1) + (R+1)
+        else d0 = hm;
+        if (i == 1)
+            fp0 = HREAL ? make_float2(H.x * pm.x, -H.x * pm.y) : cmul(H, make_float2(pm.x, -pm.y));
+        if (i == 0) p0 = pp;
+    }
+    if (x.j == 0) s.z[0].x += 1.0f - E1;
+    if (MULTI) {
+        const float2 up = make_float2(shr1(top.x), shr1(top.y));
+        const float2 dn = make_float2(shl1(d0.x), shl1(d0.y));
+        s.fp[0] = (x.j == 0) ? fp0 : up;
+        s.fm[M + R] = (x.j == a.L - 1) ? make_float2(0.f, 0.f) : dn;
+    } else {
+        s.fp[0] = fp0;
+        s.fm[M + R] = make_float2(0.f, 0.f);
+    }
+    if (TRUNC) {
+#pragma unroll
+        for (int i = 0; i < M; i++)
+            if (i == a.istar && x.j == a.jstar) s.fm[i + R + 1] = make_float2(0.f, 0.f);
+    }
+    return cmul(csave, p0);
+}
+
+template <int M, int SH>
+__device__ __forceinline__ void fm_copy_down(FusedState<M> &s)
+{
+#pragma unroll
+    for (int k = 0; k < M; k++) s.fm[k] = s.fm[k + SH];
+}
+
+template <int M, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
+__device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *__restrict__ coef,
+                                               const float *__restrict__ params,
+                                               float2 *__restrict__ atoms)
 {
     const size_t gl = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const size_t atom = MULTI ? (gl >> a.lshift) : gl;
-    const int j = MULTI ? (int)(gl & (size_t)(a.L - 1)) : 0;
+    FusedCtx x;
+    x.j = MULTI ? (int)(gl & (size_t)(a.L - 1)) : 0;
     const int slice = blockIdx.y;
     const bool valid = atom < a.natoms;
     const size_t pa = valid ? atom : 0;
     // SSFP kernels index the slice-0 parameter rows (src/functions.cu:462-463,
     // 537-538, 596): T1/T2/B0 repeat identically in every B1 slice.
-    const float T1 = a.params[pa * 4 + 0];
-    const float T2 = a.params[pa * 4 + 1];
-    const float B0 = a.params[pa * 4 + 2];
-    const float rT1 = 1.0f / T1, rT2 = 1.0f / T2;
+    const float T1 = params[pa * 4 + 0];
+    const float T2 = params[pa * 4 + 1];
+    const float B0 = params[pa * 4 + 2];
+    x.k1 = decay_k(T1);
+    x.k2 = decay_k(T2);
+    x.b0r = (double)B0 * 1e-3;
 
-    const TrCoef *cf = a.coef + (size_t)slice * a.T;
-    EpgState<M> s;
-    epg_init(s, a, j, T1, T2);
+    const TrCoef *__restrict__ cf = coef + (size_t)slice * a.T;
+    FusedState<M> s;
     {
+        EpgState<M> s0;
+        epg_init(s0, a, x.j, T1, T2);
         // to frame coordinates of TR 0
         const float2 z0 = make_float2(cf[0].zr, cf[0].zi);
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            s.fp[i] = s0.fp[i];
+            s.fm[i] = s0.fm[i];
+            s.z[i] = s0.z[i];
+        }
+#pragma unroll
+        for (int i = M; i < M + kUnroll; i++) s.fm[i] = make_float2(0.f, 0.f);
         s.fp[0] = cmulc(z0, s.fp[0]);
         s.fm[0] = cmul(z0, s.fm[0]);
     }
 
-    float2 *out = a.atoms + ((size_t)slice * a.natoms + pa) * (size_t)a.T;
-    const bool store = valid && j == 0;
+    float2 *__restrict__ out = atoms + ((size_t)slice * a.natoms + pa) * (size_t)a.T;
+    const bool store = valid && x.j == 0;
+    const bool even_t = (a.T & 1) == 0;
     double nrm = 0.0;
-
-    for (int n = 0; n < a.T; n++) {
-        const TrCoef c = cf[n];
-        const float ea = expf(-c.te * rT2);
-        const float eb = expf(-c.tmte * rT2);
-        const float E1 = expf(-c.tr * rT1);
-        float2 H, csave;
-        if (PHASE) {
-            float sa_, ca_, sb_, cb_;
-            sincosf(phase_angle(B0, c.te), &sa_, &ca_);
-            sincosf(phase_angle(B0, c.tmte), &sb_, &cb_);
-            const float2 e = cmul(make_float2(ca_, sa_), make_float2(cb_, sb_));
-            const float g = ea * eb;
-            const float2 er = cmul(e, make_float2(c.rr, c.ri));
-            H = make_float2(g * er.x, g * er.y);
-            const float2 ez = cmul(make_float2(ca_, sa_), make_float2(c.zr, c.zi));
-            csave = make_float2(ea * ez.x, ea * ez.y);
-        } else {
-            const float g = ea * eb;
-            H = HREAL ? make_float2(g, 0.f) : make_float2(g * c.rr, g * c.ri);
-            csave = make_float2(ea * c.zr, ea * c.zi);
-        }
-        const float ca2 = c.ca2, sa2 = c.sa2, sa = c.sa;
-        const float eca = E1 * c.ca, eh = E1 * 0.5f * c.sa;
-        const float rec = (j == 0) ? (1.0f - E1) : 0.0f;
-
-        float2 carry = make_float2(0.f, 0.f), p0 = carry, pm0 = carry, fp0 = carry;
-#pragma unroll
-        for (int i = 0; i < M; i++) {
-            const float2 A = s.fp[i], B = s.fm[i], C = s.z[i];
-            const float2 pp = make_float2(ca2 * A.x + sa2 * B.x + sa * C.y,
-                                          ca2 * A.y + sa2 * B.y - sa * C.x);
-            const float2 pm = make_float2(sa2 * A.x + ca2 * B.x - sa * C.y,
-                                          sa2 * A.y + ca2 * B.y + sa * C.x);
-            const float dx = A.x - B.x, dy = A.y - B.y;
-            s.z[i] = make_float2(eca * C.x + eh * dy, eca * C.y - eh * dx);
-            if (i == 0) {
-                s.z[0].x += rec;
-                p0 = pp;
-                pm0 = pm;
+    int n = 0;
+    for (; n + kUnroll <= a.T; n += kUnroll) {
+        const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+        const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
+        fm_copy_down<M, 2>(s);
+        if (store) {
+            if (even_t) {
+                *reinterpret_cast<float4 *>(out + n) = make_float4(v0.x, v0.y, v1.x, v1.y);
             } else {
-                s.fm[i - 1] = HREAL ? make_float2(H.x * pm.x, H.x * pm.y) : cmulc(H, pm);
-                s.fp[i] = carry;
+                out[n] = v0;
+                out[n + 1] = v1;
             }
-            if (i == 1)
-                fp0 = HREAL ? make_float2(H.x * pm.x, -H.x * pm.y)
-                            : cmul(H, make_float2(pm.x, -pm.y));
-            carry = HREAL ? make_float2(H.x * pp.x, H.x * pp.y) : cmul(H, pp);
+            nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
+            nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
         }
-        if (MULTI) {
-            const float2 up = make_float2(shr1(carry.x), shr1(carry.y));
-            const float2 d0 = HREAL ? make_float2(H.x * pm0.x, H.x * pm0.y) : cmulc(H, pm0);
-            const float2 dn = make_float2(shl1(d0.x), shl1(d0.y));
-            s.fp[0] = (j == 0) ? fp0 : up;
-            s.fm[M - 1] = (j == a.L - 1) ? make_float2(0.f, 0.f) : dn;
-        } else {
-            s.fp[0] = fp0;
-            s.fm[M - 1] = make_float2(0.f, 0.f);
-        }
-        epg_truncate(s, a, j);
-
-        const float2 v = cmul(csave, p0);
+    }
+    for (; n < a.T; n++) {
+        const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+        fm_copy_down<M, 1>(s);
         if (store) {
             out[n] = v;
             nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
         }
     }
     if (store && a.norm2) a.norm2[(size_t)slice * a.natoms + atom] = nrm;
+}
+
+// Two occupancy classes: without truncation the state fits 256 VGPRs
+// (2 waves per SIMD); the truncating variants get the full register file.
+template <int M, bool MULTI, bool PHASE, bool HREAL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void epg_fused_kernel(EpgArgs a, const TrCoef *__restrict__ coef,
+                      const float *__restrict__ params, float2 *__restrict__ atoms)
+{
+    epg_fused_body<M, MULTI, PHASE, HREAL, false>(a, coef, params, atoms);
+}
+
+template <int M, bool MULTI, bool PHASE, bool HREAL>
+__global__ __launch_bounds__(kBlock) void epg_fused_trunc_kernel(EpgArgs a, const TrCoef *__restrict__ coef,
+                            const float *__restrict__ params, float2 *__restrict__ atoms)
+{
+    epg_fused_body<M, MULTI, PHASE, HREAL, true>(a, coef, params, atoms);
 }
 
 // ------------------------------------------------------------------------
@@ -370,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void roa_kernel(RoaArgs a)
     const float T1 = a.params[atom * 4 + 0];
     const float T2 = a.params[atom * 4 + 1];
     const float B0 = a.params[atom * 4 + 2];
-    const float rT1 = 1.0f / T1, rT2 = 1.0f / T2;
+    const float k1 = decay_k(T1), k2 = decay_k(T2);
     float2 *out = a.atoms + atom * (size_t)a.T;
     const float inv_n = 1.0f / (float)a.n_iso;
     double nrm = 0.0;
@@ -378,13 +497,14 @@ __global__ __launch_bounds__(kBlock) void roa_kernel(RoaArgs a)
     float sa0, ca0;
     sincosf((float)(180.0 * M_PI / 180.0), &sa0, &ca0);
     for (int q0 = 0; q0 < a.n_iso; q0 += NI) {
-        float mx[NI], my[NI], mz[NI], b0q[NI];
+        float mx[NI], my[NI], mz[NI];
+        double b0q[NI];   // isochromat off-resonance in revolutions per (2 ms)
 #pragma unroll
         for (int u = 0; u < NI; u++) {
             const int q = q0 + u;
             const double off = (a.n_iso == 1)
                 ? 0.0 : (double)a.spread * (((double)q + 0.5) / (double)a.n_iso - 0.5);
-            b0q[u] = (float)((double)B0 + off);
+            b0q[u] = (double)(float)((double)B0 + off) / 2000.0;
             mx[u] = 0.f;
             my[u] = sa0;
             mz[u] = ca0;
@@ -392,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void roa_kernel(RoaArgs a)
         const bool last = q0 + NI >= a.n_iso;
         for (int n = 0; n < a.T; n++) {
             const TrCoef c = a.coef[n];
-            const float e1 = expf(-c.tr * rT1), e2 = expf(-c.tr * rT2);
+            const float e1 = decay(c.tr, k1), e2 = decay(c.tr, k2);
             float sx = 0.f, sy = 0.f;
 #pragma unroll
             for (int u = 0; u < NI; u++) {
@@ -404,10 +524,9 @@ __global__ __launch_bounds__(kBlock) void roa_kernel(RoaArgs a)
                     // decay over TR (:1094-1105)
                     float x = e2 * nx, y = e2 * ny;
                     mz[u] = e1 * nz + (1.f - e1);
-                    // half-TR precession (:1107-1143); beta is a float
-                    const float beta = (float)(M_PI * (double)b0q[u] * (double)c.tr / 1000.0);
+                    // half-TR precession pi*B0*TR/1000 rad = B0*TR/2000 rev (:1107-1143)
                     float sb, cb;
-                    sincosf(beta, &sb, &cb);
+                    sincos_rev(b0q[u] * (double)c.tr, &sb, &cb);
                     float x2 = cb * x + sb * y, y2 = cb * y - sb * x;
                     sx += x2;
                     sy += y2;
@@ -455,16 +574,23 @@ int launch_m(const EpgLaunch &l, const EpgArgs &args, dim3 grid, hipStream_t s)
     const bool multi = args.L > 1;
     const bool phase = l.echo == kEpgSsfp;
     if (!l.generic && (l.echo == kEpgSsfp || l.echo == kEpgFisp)) {
-        if (phase) {
-            if (multi) epg_fused_kernel<M, true, true, false><<<grid, kBlock, 0, s>>>(args);
-            else epg_fused_kernel<M, false, true, false><<<grid, kBlock, 0, s>>>(args);
-        } else if (l.h_real) {
-            if (multi) epg_fused_kernel<M, true, false, true><<<grid, kBlock, 0, s>>>(args);
-            else epg_fused_kernel<M, false, false, true><<<grid, kBlock, 0, s>>>(args);
-        } else {
-            if (multi) epg_fused_kernel<M, true, false, false><<<grid, kBlock, 0, s>>>(args);
-            else epg_fused_kernel<M, false, false, false><<<grid, kBlock, 0, s>>>(args);
-        }
+        const bool trunc = args.istar >= 0;
+#define A3 args.coef, args.params, args.atoms
+#define MRF_FUSED(PH, HR)                                                                      \
+    do {                                                                                       \
+        if (multi) {                                                                           \
+            if (trunc) epg_fused_trunc_kernel<M, true, PH, HR><<<grid, kBlock, 0, s>>>(args, A3);   \
+            else epg_fused_kernel<M, true, PH, HR><<<grid, kBlock, 0, s>>>(args, A3);               \
+        } else {                                                                               \
+            if (trunc) epg_fused_trunc_kernel<M, false, PH, HR><<<grid, kBlock, 0, s>>>(args, A3);  \
+            else epg_fused_kernel<M, false, PH, HR><<<grid, kBlock, 0, s>>>(args, A3);              \
+        }                                                                                      \
+    } while (0)
+        if (phase) MRF_FUSED(true, false);
+        else if (l.h_real) MRF_FUSED(false, true);
+        else MRF_FUSED(false, false);
+#undef MRF_FUSED
+#undef A3
         return 0;
     }
     if (M != 8 && M != 32) return -1;   // reference-order path: two lane widths

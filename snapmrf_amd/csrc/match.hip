@@ -63,8 +63,27 @@ __device__ __forceinline__ unsigned long long make_key(float score, uint32_t idx
     return ((unsigned long long)__float_as_uint(score) << 32) | (unsigned long long)(~idx);
 }
 
+// Two consecutive complex values (t, t+1) of a row, zero outside [0, T) or
+// for a null row.  Rows are 16-byte aligned only when T is even.
+__device__ __forceinline__ float4 load_pair(const float2 *row, int t, int T, bool even_t)
+{
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!row) return v;
+    if (t + 1 < T) {
+        if (even_t) return *reinterpret_cast<const float4 *>(row + t);
+        const float2 x = row[t], y = row[t + 1];
+        return make_float4(x.x, x.y, y.x, y.y);
+    }
+    if (t < T) {
+        const float2 x = row[t];
+        v = make_float4(x.x, x.y, 0.f, 0.f);
+    }
+    return v;
+}
+
 __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
 {
+    const bool even_t = (a.T & 1) == 0;
     const int tile_id = blockIdx.x;
     if (tile_id >= *a.ntiles) return;
     const Tile tile = a.tiles[tile_id];
@@ -131,17 +150,8 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
             float4 ga[4], gb[4];
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
-                if (arow[p]) {
-                    if (tc + 1 < a.T) va = *reinterpret_cast<const float4 *>(arow[p] + tc);
-                    else if (tc < a.T) { float2 t = arow[p][tc]; va = make_float4(t.x, t.y, 0.f, 0.f); }
-                }
-                if (vrow[p]) {
-                    if (tc + 1 < a.T) vb = *reinterpret_cast<const float4 *>(vrow[p] + tc);
-                    else if (tc < a.T) { float2 t = vrow[p][tc]; vb = make_float4(t.x, t.y, 0.f, 0.f); }
-                }
-                ga[p] = va;
-                gb[p] = vb;
+                ga[p] = load_pair(arow[p], tc, a.T, even_t);
+                gb[p] = load_pair(vrow[p], tc, a.T, even_t);
             }
             __syncthreads();
 #pragma unroll

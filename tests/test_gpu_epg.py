@@ -138,3 +138,16 @@ def test_bad_arguments_raise(gpu):
         sm.MRF_dict(d_atoms, d_params, raw, 5, 5, 4, 3, 4, 101, "epg_ssfp")   # NATOMS % natoms
     with pytest.raises(sm.MRFError):
         sm.MRF_dict(d_atoms, d_params, raw, 5, 5, 4, 4, 4, 1, "epg_ssfp")     # w < 2
+
+
+@pytest.mark.parametrize("echo,w,T", [("epg_ssfp", 17, 333), ("epg_fisp", 7, 101),
+                                      ("epg_ssfp", 2, 9), ("epg_ssfp", 33, 64)])
+def test_truncated_and_odd_lengths(gpu, echo, w, T):
+    """w not equal to a compiled lane width (padded states, truncation at w-1)
+    and odd T (the TR loop's single-step tail)."""
+    s = synth.sequence(T, seed=w)
+    raw = synth.csv_columns(s, echo)
+    params, A, natoms = sm.build_params("150:350:2000", "25:45:300", "-35:35:35", "1")
+    got = gen(gpu, echo, raw, params, natoms, w)
+    ref = oracle.epg(echo, oracle.seq_prepare(echo, raw), params, natoms, w)
+    assert rel_err(got, ref) < TOL

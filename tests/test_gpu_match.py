@@ -63,14 +63,25 @@ def _fisp_dict(torch, T=1000, seed=1):
     return d_atoms.cpu().numpy(), params, natoms
 
 
+def fp64_score(atoms, sig, idx):
+    """|<a_idx, s>| in FP64 for each voxel's chosen atom."""
+    a = atoms[idx].astype(np.complex128)
+    return np.abs(np.sum(np.conj(a) * sig.astype(np.complex128), axis=1))
+
+
 def _check_against_oracle(maps, idx, atoms, params, sig, base=0):
+    """Exact index agreement on voxels whose FP64 top-2 gap exceeds the FP32
+    bound; on every voxel the GPU's pick scores within that bound of the
+    FP64 best.  Returns the number of near-tie voxels."""
     ref_idx, ref_pd, ref_sec = oracle.match_slice(atoms, sig)
     gap = (ref_pd - ref_sec) / np.maximum(ref_pd, 1e-30)
     clear = gap > GAP
-    assert clear.mean() > 0.5
     assert np.array_equal(idx[clear], ref_idx[clear] + base)
-    assert np.allclose(maps[:, 4], ref_pd, rtol=1e-5, atol=1e-7)
     assert np.array_equal(maps[clear, :4], params[ref_idx[clear] + base])
+    picked = fp64_score(atoms, sig, idx - base)
+    assert np.all(picked >= ref_pd * (1.0 - GAP))
+    assert np.array_equal(maps[:, :4], params[idx])
+    assert np.allclose(maps[:, 4], ref_pd, rtol=1e-5, atol=1e-7)
     return int((~clear).sum())
 
 
