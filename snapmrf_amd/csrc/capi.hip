@@ -182,9 +182,11 @@ extern "C" int mrf_dict_ex(void *stream, const char *echo_type, const float *h_m
         l.echo = echo;
         l.generic = opts && opts->reserved[0] == 1;
         l.h_real = h_real && echo == kEpgFisp;
+        l.normalize = normalize != 0;
         rc = launch_epg(l, s);
     }
-    if (!rc && normalize) rc = launch_normalize(reinterpret_cast<float2 *>(d_atoms), d_n2, n_gen, T, s);
+    if (!rc && normalize && echo == kRoaSsfp)
+        rc = launch_normalize(reinterpret_cast<float2 *>(d_atoms), d_n2, n_gen, T, s);
     (void)hipFreeAsync(d_seq, s);
     (void)hipFreeAsync(d_coef, s);
     if (own_n2) (void)hipFreeAsync(d_n2, s);

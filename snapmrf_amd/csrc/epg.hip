@@ -52,6 +52,7 @@ struct EpgArgs {
     int istar;      // truncation index of state w-1 inside its lane, -1 if none
     int jstar;      // lane of state w-1 inside the group
     float init_alpha, init_phi, TI;   // initial pulse (float32 radians / ms)
+    int normalize;  // scale each atom to unit L2 norm inside the kernel
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
@@ -261,6 +262,92 @@ __device__ __forceinline__ void fm_copy_down(FusedState<M> &s)
     for (int k = 0; k < M; k++) s.fm[k] = s.fm[k + SH];
 }
 
+// Output staging for one-lane-per-atom waves.  Each lane's samples go to its
+// row of a per-wave LDS tile; every kSeg TRs the wave writes its 64 atom
+// rows as whole 128-byte segments (8 lanes x 16 bytes per row), so every
+// HBM line is written once and completely.
+constexpr int kSeg = 16;
+constexpr int kSegStride = 2 * kSeg + 2;   // floats per LDS row (padding spreads banks)
+
+struct WaveStage {
+    float *lds;        // this wave's [64][kSegStride] floats
+    float2 *rows;      // first atom row of this wave
+    int T;
+    int nvalid;        // atom rows of this wave that exist
+    int lane;
+};
+
+__device__ __forceinline__ void stage_put(const WaveStage &w, int slot, float2 v)
+{
+    *reinterpret_cast<float2 *>(&w.lds[w.lane * kSegStride + 2 * slot]) = v;
+}
+
+// Writes TRs [t0, t0 + count) of the wave's rows.
+__device__ __forceinline__ void stage_flush(const WaveStage &w, int t0, int count, bool even_t)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int p = w.lane & 7;
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+        const int row = it * 8 + (w.lane >> 3);
+        if (row < w.nvalid) {
+            const float *src = &w.lds[row * kSegStride + 4 * p];
+            float2 *dst = w.rows + (size_t)row * w.T + t0 + 2 * p;
+            const float2 a = *reinterpret_cast<const float2 *>(src);
+            const float2 b = *reinterpret_cast<const float2 *>(src + 2);
+            if (2 * p + 1 < count) {
+                if (even_t) *reinterpret_cast<float4 *>(dst) = make_float4(a.x, a.y, b.x, b.y);
+                else {
+                    dst[0] = a;
+                    dst[1] = b;
+                }
+            } else if (2 * p < count) {
+                dst[0] = a;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float4 scale_v(float4 v, float f)
+{
+    return make_float4(v.x * f, v.y * f, v.z * f, v.w * f);
+}
+__device__ __forceinline__ float2 scale_v(float2 v, float f) { return make_float2(v.x * f, v.y * f); }
+
+// Scales the wave's nvalid contiguous rows by the factors in w.lds[row];
+// VEC elements, per_row of them per row.
+template <typename VEC>
+__device__ __forceinline__ void normalize_rows(const WaveStage &w, int per_row)
+{
+    VEC *base = reinterpret_cast<VEC *>(w.rows);
+    const int total = w.nvalid * per_row;
+    const float inv = 1.0f / (float)per_row;
+    constexpr int kDepth = 32;   // loads in flight per lane
+    for (int i0 = w.lane; i0 < total; i0 += 64 * kDepth) {
+        VEC v[kDepth];
+#pragma unroll
+        for (int u = 0; u < kDepth; u++) {
+            const int idx = i0 + 64 * u;
+            if (idx < total) v[u] = base[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < kDepth; u++) {
+            const int idx = i0 + 64 * u;
+            if (idx < total) {
+                int r = (int)((float)idx * inv);
+                r += (r + 1) * per_row <= idx;
+                r -= r * per_row > idx;
+                base[idx] = scale_v(v[u], w.lds[r]);
+            }
+        }
+    }
+}
+
 template <int M, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
 __device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *__restrict__ coef,
                                                const float *__restrict__ params,
@@ -306,27 +393,73 @@ __device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *_
     const bool even_t = (a.T & 1) == 0;
     double nrm = 0.0;
     int n = 0;
-    for (; n + kUnroll <= a.T; n += kUnroll) {
-        const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
-        const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
-        fm_copy_down<M, 2>(s);
-        if (store) {
-            if (even_t) {
-                *reinterpret_cast<float4 *>(out + n) = make_float4(v0.x, v0.y, v1.x, v1.y);
-            } else {
-                out[n] = v0;
-                out[n + 1] = v1;
-            }
+    if constexpr (!MULTI) {
+        __shared__ __attribute__((aligned(16))) float stage_lds[kBlock / 64][64 * kSegStride];
+        const size_t first = (size_t)blockIdx.x * kBlock + (threadIdx.x & ~63u);
+        WaveStage w;
+        w.lds = stage_lds[threadIdx.x >> 6];
+        w.rows = atoms + ((size_t)slice * a.natoms + first) * (size_t)a.T;
+        w.T = a.T;
+        w.nvalid = first < a.natoms ? (int)((a.natoms - first) < 64 ? (a.natoms - first) : 64) : 0;
+        w.lane = threadIdx.x & 63;
+        int t0 = 0;
+        for (; n + kUnroll <= a.T; n += kUnroll) {
+            const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+            const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
+            fm_copy_down<M, 2>(s);
+            stage_put(w, n - t0, v0);
+            stage_put(w, n + 1 - t0, v1);
             nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
             nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
+            if (n + 2 - t0 == kSeg) {
+                stage_flush(w, t0, kSeg, even_t);
+                t0 += kSeg;
+            }
         }
-    }
-    for (; n < a.T; n++) {
-        const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
-        fm_copy_down<M, 1>(s);
-        if (store) {
-            out[n] = v;
+        for (; n < a.T; n++) {
+            const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+            fm_copy_down<M, 1>(s);
+            stage_put(w, n - t0, v);
             nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+        if (a.T > t0) stage_flush(w, t0, a.T - t0, even_t);
+        if (a.normalize) {
+            // normalize (src/functions.cu:670-685) on the wave's own rows,
+            // which are contiguous: per-row factors go to LDS, then the wave
+            // streams the block with 32 independent loads in flight per lane.
+            // (the wave re-reads its own stores through its CU's caches:
+            // workgroup scope suffices and avoids an L2 write-back)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            w.lds[w.lane] = nrm > 0.0 ? (float)(1.0 / sqrt(nrm)) : 1.0f;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (even_t) normalize_rows<float4>(w, a.T / 2);
+            else normalize_rows<float2>(w, a.T);
+        }
+    } else {
+        for (; n + kUnroll <= a.T; n += kUnroll) {
+            const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+            const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
+            fm_copy_down<M, 2>(s);
+            if (store) {
+                if (even_t) {
+                    *reinterpret_cast<float4 *>(out + n) = make_float4(v0.x, v0.y, v1.x, v1.y);
+                } else {
+                    out[n] = v0;
+                    out[n + 1] = v1;
+                }
+                nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
+                nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
+            }
+        }
+        for (; n < a.T; n++) {
+            const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+            fm_copy_down<M, 1>(s);
+            if (store) {
+                out[n] = v;
+                nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            }
         }
     }
     if (store && a.norm2) a.norm2[(size_t)slice * a.natoms + atom] = nrm;
@@ -657,6 +790,7 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
     args.init_alpha = (float)((double)a_deg * M_PI / 180.0);
     args.init_phi = (float)((double)p_deg * M_PI / 180.0);
     args.TI = TI;
+    args.normalize = (l.normalize && fused && L == 1) ? 1 : 0;
 
     const size_t lanes = l.natoms * (size_t)L;
     const size_t blocks = (lanes + kBlock - 1) / kBlock;
@@ -678,6 +812,8 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
     if (rc) return mrf_fail(MRF_E_UNSUPPORTED, "no EPG kernel for w=%d", l.w);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "EPG launch: %s", hipGetErrorString(e));
+    if (l.normalize && !args.normalize)
+        return launch_normalize(l.atoms, l.norm2, l.natoms * (size_t)l.n_slices, l.T, s);
     return 0;
 }
 

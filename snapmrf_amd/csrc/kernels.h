@@ -24,6 +24,7 @@ struct EpgLaunch {
     int echo;              // Echo
     bool generic;          // force the unfused reference-order path
     bool h_real;           // constant RF phase and no B0 rotation (FISP)
+    bool normalize;        // unit L2 norm per atom (norm2 must be non-null)
 };
 
 // EPG dictionary: SSFP/FISP fused path or SR/SE/TSE reference-order path.

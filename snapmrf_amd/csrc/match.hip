@@ -144,15 +144,15 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
                 acc_im[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
 
-        for (int c = 0; c < nchunks; c++) {
-            // global -> registers (zero outside [0, T) and outside the tile)
-            const int tc = c * KC + (st_col >> 1);   // first timepoint of this float4
-            float4 ga[4], gb[4];
+        // global -> registers (zero outside [0, T) and outside the tile); the
+        // next chunk's loads are issued before the current chunk's MFMAs
+        float4 ga[4], gb[4];
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                ga[p] = load_pair(arow[p], tc, a.T, even_t);
-                gb[p] = load_pair(vrow[p], tc, a.T, even_t);
-            }
+        for (int p = 0; p < 4; p++) {
+            ga[p] = load_pair(arow[p], st_col >> 1, a.T, even_t);
+            gb[p] = load_pair(vrow[p], st_col >> 1, a.T, even_t);
+        }
+        for (int c = 0; c < nchunks; c++) {
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < 4; p++) {
@@ -160,6 +160,14 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
                 *reinterpret_cast<float4 *>(&lds_b[(st_row + 32 * p) * LS + st_col]) = gb[p];
             }
             __syncthreads();
+            if (c + 1 < nchunks) {
+                const int tc = (c + 1) * KC + (st_col >> 1);
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    ga[p] = load_pair(arow[p], tc, a.T, even_t);
+                    gb[p] = load_pair(vrow[p], tc, a.T, even_t);
+                }
+            }
 #pragma unroll
             for (int q = 0; q < KF / 16; q++) {
                 floatx4 fa[4], fx[4], fy[4];
@@ -253,11 +261,26 @@ __global__ void b1_select_kernel(const float *b1map, const float *params, size_t
     vox_slice[v] = m;
 }
 
-__global__ void count_kernel(const int32_t *vox_slice, size_t V, int *counts)
+// Slice histogram: per-block LDS counts, one global atomic per (block, slice).
+constexpr int kMaxLdsSlices = 1024;
+
+__global__ void count_kernel(const int32_t *vox_slice, size_t V, int n_slices, int *counts)
 {
+    __shared__ int h[kMaxLdsSlices];
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    atomicAdd(&counts[vox_slice ? vox_slice[v] : 0], 1);
+    const bool lds = n_slices <= kMaxLdsSlices;
+    if (lds)
+        for (int k = threadIdx.x; k < n_slices; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    if (v < V) {
+        const int k = vox_slice ? vox_slice[v] : 0;
+        if (lds) atomicAdd(&h[k], 1);
+        else atomicAdd(&counts[k], 1);
+    }
+    __syncthreads();
+    if (lds)
+        for (int k = threadIdx.x; k < n_slices; k += blockDim.x)
+            if (h[k]) atomicAdd(&counts[k], h[k]);
 }
 
 // single work-item: exclusive scan of the slice counts and the tile list
@@ -282,14 +305,27 @@ __global__ void tiles_kernel(const int *counts, int n_slices, int *offsets, Tile
     *ntiles = nt;
 }
 
-__global__ void scatter_kernel(const int32_t *vox_slice, size_t V, const int *offsets,
-                               int *fill, int32_t *perm)
+// Voxel permutation grouped by slice: ranks within the block from LDS
+// atomics, one global reservation per (block, slice).
+__global__ void scatter_kernel(const int32_t *vox_slice, size_t V, int n_slices,
+                               const int *offsets, int *fill, int32_t *perm)
 {
+    __shared__ int h[kMaxLdsSlices];
+    __shared__ int base[kMaxLdsSlices];
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const int k = vox_slice ? vox_slice[v] : 0;
-    const int p = offsets[k] + atomicAdd(&fill[k], 1);
-    perm[p] = (int32_t)v;
+    const int k = (v < V && vox_slice) ? vox_slice[v] : 0;
+    if (n_slices > kMaxLdsSlices) {
+        if (v < V) perm[offsets[k] + atomicAdd(&fill[k], 1)] = (int32_t)v;
+        return;
+    }
+    for (int q = threadIdx.x; q < n_slices; q += blockDim.x) h[q] = 0;
+    __syncthreads();
+    const int rank = (v < V) ? atomicAdd(&h[k], 1) : 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < n_slices; q += blockDim.x)
+        if (h[q]) base[q] = offsets[q] + atomicAdd(&fill[q], h[q]);
+    __syncthreads();
+    if (v < V) perm[base[k] + rank] = (int32_t)v;
 }
 
 __global__ void maps_kernel(const unsigned long long *keys, const float *params, size_t V,
@@ -391,15 +427,12 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     Scratch sc = carve(scratch, l);
     const unsigned vb = (unsigned)((l.V + 255) / 256);
     zero_u64_kernel<<<vb, 256, 0, s>>>(sc.keys, l.V);
-    zero_i32_kernel<<<1, 256, 0, s>>>(sc.counts, (size_t)ns);
-    zero_i32_kernel<<<1, 256, 0, s>>>(sc.fill, (size_t)ns);
-    if (ns > 256) {
-        zero_i32_kernel<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(sc.counts, (size_t)ns);
-        zero_i32_kernel<<<(unsigned)((ns + 255) / 256), 256, 0, s>>>(sc.fill, (size_t)ns);
-    }
-    count_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, sc.counts);
+    const unsigned nb = (unsigned)((ns + 255) / 256);
+    zero_i32_kernel<<<nb, 256, 0, s>>>(sc.counts, (size_t)ns);
+    zero_i32_kernel<<<nb, 256, 0, s>>>(sc.fill, (size_t)ns);
+    count_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, ns, sc.counts);
     tiles_kernel<<<1, 1, 0, s>>>(sc.counts, ns, sc.offsets, sc.tiles, sc.ntiles);
-    scatter_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, sc.offsets, sc.fill, sc.perm);
+    scatter_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, ns, sc.offsets, sc.fill, sc.perm);
 
     MatchArgs a{};
     a.atoms = l.atoms;
