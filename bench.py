@@ -67,21 +67,21 @@ def cpu_baseline(seq_raw, params, natoms, T, w, budget_s=8.0):
                 match_value=vox_per_s, match_unit="voxels/s")
 
 
-def make_signals(torch, np, synth, d_atoms, params, natoms, T, nx=240, ny=240, snr=25.0):
+def make_signals(torch, np, synth, d_atoms, params, natoms, T, nx=240, ny=240, snr=25.0, seed=0):
     """240x240 voxels: atom of the slice nearest the smooth B1 map value,
     PD ~ U(0.3, 1), complex Gaussian noise at the given SNR (on the GPU)."""
     V = nx * ny
-    b1 = synth.b1_map(nx, ny, seed=0).ravel()
+    b1 = synth.b1_map(nx, ny, seed=seed).ravel()
     slice_b1 = params[::natoms, 3]
     d = np.abs(b1[:, None] - slice_b1[None, :])
     ks = np.argmin(np.where(d < 10.0, d, 10.0), axis=1)
-    rng = np.random.default_rng(1)
+    rng = np.random.default_rng(1 + 1000 * seed)
     truth = ks * natoms + rng.integers(0, natoms, V)
     pd = rng.uniform(0.3, 1.0, V).astype(np.float32)
     d_truth = torch.from_numpy(truth).cuda()
     clean = d_atoms[d_truth] * torch.from_numpy(pd).cuda()[:, None]
     rms = clean.abs().pow(2).mean(dim=1).sqrt()
-    g = torch.Generator(device="cuda").manual_seed(2)
+    g = torch.Generator(device="cuda").manual_seed(2 + 1000 * seed)
     noise = torch.complex(torch.randn(V, T, device="cuda", generator=g),
                           torch.randn(V, T, device="cuda", generator=g))
     d_sig = (clean + noise * (rms / (snr * 2 ** 0.5))[:, None]).to(torch.complex64).contiguous()
@@ -102,12 +102,114 @@ def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_
                 d_b1map=d_b1, stream=stream)
     torch.cuda.synchronize()
     log("profile: matching done")
+    del d_sig, d_b1, d_maps
+    torch.cuda.empty_cache()
+
+    def timed1(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        return 0.0
+    precision_form(sm, torch, np, synth, raw, T, w, stream, 1, timed1, 0)
+    log("profile: precision form done")
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
     log("profile: 1 GiB calibration copy done")
     return 0
+
+
+GRID_LOWP = ("100:20:2000+2300:300:5000", "20:5:100+110:10:300+400:200:2000", "-40:4:40",
+             "0.8:0.1:1.2")
+PEAK_BF16 = 2500.0         # TFLOP/s, MI355X dense BF16 MFMA
+
+
+def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle_voxels=256):
+    """BF16 storage + BF16 MFMA top-8 + FP32 rescore, and split-BF16, against
+    the FP32 matcher on 240x240x8 voxels; map errors against the FP64 oracle
+    on a seeded voxel subset."""
+    params, A, natoms = sm.build_params(*GRID_LOWP)
+    log(f"precision form: {A} atoms ({A // natoms} slices x {natoms}), 460800 voxels")
+    d_params = torch.from_numpy(params).cuda()
+    d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms, stream=stream)
+    d_hi = torch.empty((A, T), dtype=torch.int32, device="cuda")
+    d_lo = torch.empty((A, T), dtype=torch.int32, device="cuda")
+    prep_ms = timed(lambda: sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo, stream=stream), 1)
+    sigs, b1s, truths = [], [], []
+    for sl in range(8):
+        d_sig, d_b1, truth, V1 = make_signals(torch, np, synth, d_atoms, params, natoms, T,
+                                              seed=sl)
+        sigs.append(d_sig)
+        b1s.append(d_b1)
+        truths.append(truth)
+    d_sig = torch.cat(sigs)
+    d_b1 = torch.cat(b1s)
+    truth = np.concatenate(truths)
+    del sigs, b1s
+    V = d_sig.shape[0]
+    res = {"workload": f"epg_ssfp {A} atoms (B0 -40:4:40) x T={T}, 240x240x8 = {V} voxels, "
+                       f"B1-map slice selection", "atoms": A, "voxels": V,
+           "dictionary_bytes": {"c64": A * T * 8, "bf16": A * T * 4, "split_bf16": A * T * 8},
+           "lowp_prepare_ms": prep_ms}
+    out = {}
+    for name, mode, k in (("fp32", sm.MRF_MATCH_FP32, 1), ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
+                          ("split3", sm.MRF_MATCH_SPLIT3, steps)):
+        d_maps = torch.empty((V, 5), device="cuda")
+        d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+
+        def run():
+            sm.match_ex(mode, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps, d_b1map=d_b1,
+                        d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo, stream=stream)
+        run()
+        torch.cuda.synchronize()
+        ms = timed(run, k)
+        out[name] = (d_maps.cpu().numpy(), d_idx.cpu().numpy())
+        flops = 8.0 * T * natoms * V
+        res[name] = {"voxels_per_s": V / (ms * 1e-3), "ms_per_call": ms, "calls": k,
+                     "tflops_algorithmic": flops / (ms * 1e-3) / 1e12,
+                     "truth_agree": float(np.mean(out[name][1] == truth))}
+        log(f"precision form {name}: {ms:.1f} ms/call")
+    i32 = out["fp32"][1]
+    for name in ("bf16_top8", "split3"):
+        res[name]["argmax_agree_fp32"] = float(np.mean(out[name][1] == i32))
+        res[name]["speedup_vs_fp32"] = res[name]["voxels_per_s"] / res["fp32"]["voxels_per_s"]
+    res["roofline_bf16"] = {"bound": "mfma", "achieved": res["bf16_top8"]["tflops_algorithmic"],
+                            "peak": PEAK_BF16, "unit": "TFLOP/s",
+                            "frac": res["bf16_top8"]["tflops_algorithmic"] / PEAK_BF16,
+                            "traffic": measured_traffic("match_bf16"),
+                            "kernel": "match_lowp_kernel<false,32> + rescore_kernel"}
+    # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path
+    if oracle_voxels:
+        import oracle
+        rng = np.random.default_rng(7)
+        sub = np.sort(rng.choice(V, oracle_voxels, replace=False))
+        sig = d_sig[torch.from_numpy(sub).cuda()].cpu().numpy()
+        b1 = d_b1.cpu().numpy()[sub]
+        atoms = d_atoms.cpu().numpy()
+        slice_b1 = params[::natoms, 3]
+        ref = np.zeros((oracle_voxels, 5))
+        t0 = time.time()
+        for k in range(A // natoms):
+            sel = np.array([oracle.b1_slice(v, slice_b1) == k for v in b1])
+            if not sel.any():
+                continue
+            ridx, rpd, _ = oracle.match_slice(atoms[k * natoms:(k + 1) * natoms], sig[sel])
+            ref[sel, :4] = params[ridx + k * natoms]
+            ref[sel, 4] = rpd
+        log(f"FP64 oracle on {oracle_voxels} voxels: {time.time() - t0:.1f} s")
+        errs = {}
+        for name in ("fp32", "bf16_top8", "split3"):
+            m = out[name][0][sub]
+            errs[name] = {"T1_mae_ms": float(np.mean(np.abs(m[:, 0] - ref[:, 0]))),
+                          "T2_mae_ms": float(np.mean(np.abs(m[:, 1] - ref[:, 1]))),
+                          "B0_mae_hz": float(np.mean(np.abs(m[:, 2] - ref[:, 2]))),
+                          "PD_max_rel": float(np.max(np.abs(m[:, 4] - ref[:, 4]) / ref[:, 4])),
+                          "map_exact": float(np.mean(np.all(m[:, :4] == ref[:, :4], axis=1)))}
+        res["fp64_oracle_subset"] = {"voxels": oracle_voxels, "errors": errs}
+    del d_atoms, d_hi, d_lo, d_sig
+    torch.cuda.empty_cache()
+    return res
 
 
 def measured_traffic(key: str):
@@ -130,9 +232,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--small", action="store_true", help="reduced grid (debugging only)")
     ap.add_argument("--skip-match", action="store_true")
+    ap.add_argument("--skip-lowp", action="store_true", help="skip the BF16 precision form")
+    ap.add_argument("--oracle-voxels", type=int, default=256,
+                    help="voxels of the precision form checked against the FP64 oracle")
     ap.add_argument("--profile", action="store_true",
-                    help="one EPG launch, one matching call and a 1 GiB device copy (the "
-                         "HBM-counter calibration), for rocprofv3 runs; prints no JSON")
+                    help="one EPG launch, one FP32 matching call, the precision-form matchers "
+                         "and a 1 GiB device copy (the HBM-counter calibration), for rocprofv3 "
+                         "runs; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the whole path on a single "
@@ -236,6 +342,13 @@ def main() -> int:
                                  "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
                                  "algorithmic_flops_per_launch": mflops}
+    if not args.skip_lowp:
+        if not args.skip_match:
+            del d_sig, d_b1, d_maps, d_idx
+        del d_atoms, d_norm2
+        torch.cuda.empty_cache()
+        out["precision_form"] = precision_form(sm, torch, np, synth, raw, T, w, stream,
+                                               args.steps, timed, args.oracle_voxels)
     if not args.no_cpu:
         log("CPU baseline (FP64 oracle on host cores)...")
         cb = cpu_baseline(raw, params, natoms, T, w)

@@ -162,3 +162,117 @@ def test_pd_merge_is_global_first_index_argmax(gpu):
     sig, truth, pd = synth.signals_np(atoms, 1000, seed=4, snr=20.0)
     maps, idx = match(gpu, atoms, params, sig, natoms)
     _check_against_oracle(maps, idx, atoms, params, sig)
+
+
+# --------------------------------------------------------------- BF16 ----
+
+def match_lowp(torch, atoms, params, sig, natoms, mode, b1map=None):
+    """BF16 (top-8 + FP32 rescore) or split-BF16 matching through the C ABI."""
+    A, T = atoms.shape
+    d_atoms = dev(torch, atoms)
+    d_hi = torch.empty((A, T), dtype=torch.int32, device="cuda")
+    d_lo = torch.empty((A, T), dtype=torch.int32, device="cuda") if mode == sm.MRF_MATCH_SPLIT3 else None
+    sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo)
+    V = sig.shape[0]
+    d_params, d_sig = dev(torch, params), dev(torch, sig)
+    d_maps = torch.zeros((V, 5), device="cuda")
+    d_idx = torch.zeros(V, dtype=torch.int32, device="cuda")
+    d_b1 = None if b1map is None else dev(torch, b1map.astype(np.float32))
+    sm.match_ex(mode, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps, d_b1map=d_b1,
+                d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+def test_lowp_prepare_rounding(gpu):
+    """hi = bf16 round-to-nearest-even of x, lo = bf16(x - hi): checked
+    against numpy's bit arithmetic."""
+    rng = np.random.default_rng(0)
+    x = (rng.normal(size=(7, 33)) + 1j * rng.normal(size=(7, 33))).astype(np.complex64)
+    x[0, :4] = [0, 1e-30, -2.5, 3.0000002]
+    d = dev(gpu, x)
+    hi = gpu.empty((7, 33), dtype=gpu.int32, device="cuda")
+    lo = gpu.empty((7, 33), dtype=gpu.int32, device="cuda")
+    sm.lowp_prepare(d, 33, 7, hi, lo)
+    h = hi.cpu().numpy().view(np.uint32)
+    l = lo.cpu().numpy().view(np.uint32)
+
+    def rne(f):
+        u = f.view(np.uint32).astype(np.uint64)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32)
+
+    def tof(b):
+        return (b.astype(np.uint32) << 16).view(np.float32)
+
+    re = x.real.astype(np.float32)
+    im = x.imag.astype(np.float32)
+    assert np.array_equal(h & 0xFFFF, rne(re))
+    assert np.array_equal(h >> 16, rne(im))
+    assert np.array_equal(l & 0xFFFF, rne((re - tof(rne(re))).astype(np.float32)))
+    # hi + lo reproduces x to ~2^-16 relative
+    rec = tof(h & 0xFFFF) + tof(l & 0xFFFF)
+    assert np.all(np.abs(rec - re) <= np.abs(re) * 2.0 ** -15 + 1e-38)
+
+
+@pytest.mark.parametrize("mode", [sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_SPLIT3])
+def test_lowp_agrees_with_fp32_and_oracle(gpu, mode):
+    """B1-sliced SSFP dictionary, noisy voxels: the rescored argmax agrees
+    with the FP32 matcher on >= 99.9% of voxels and equals the FP64 oracle on
+    every voxel whose top-2 gap exceeds the FP32 bound (the top-8 recall)."""
+    atoms, params, natoms = _ssfp_dict(gpu, T=400)
+    sig, truth, pd = synth.signals_np(atoms, 3000, seed=12, snr=25.0)
+    m32, i32 = match(gpu, atoms, params, sig, natoms)
+    mlo, ilo = match_lowp(gpu, atoms, params, sig, natoms, mode)
+    assert np.mean(ilo == i32) >= 0.999
+    ref_idx, ref_pd, ref_sec = oracle.match_slice(atoms, sig)
+    clear = (ref_pd - ref_sec) / ref_pd > GAP
+    assert np.mean(ilo[clear] == ref_idx[clear]) >= 0.999
+    picked = fp64_score(atoms, sig, ilo)
+    assert np.mean(picked >= ref_pd * (1 - GAP)) >= 0.999
+    assert np.array_equal(mlo[:, :4], params[ilo])
+    ok = ilo == i32
+    assert np.allclose(mlo[ok, 4], m32[ok, 4], rtol=2e-5)
+
+
+def test_lowp_b1_map_mode(gpu):
+    atoms, params, natoms = _ssfp_dict(gpu)
+    b1 = synth.b1_map(30, 40, seed=8).ravel()
+    slice_b1 = params[::natoms, 3]
+    ks = np.array([oracle.b1_slice(v, slice_b1) for v in b1])
+    rng = np.random.default_rng(2)
+    pick = ks * natoms + rng.integers(0, natoms, len(b1))
+    sig = (atoms[pick] * 0.6).astype(np.complex64)
+    m32, i32 = match(gpu, atoms, params, sig, natoms, b1map=b1)
+    mlo, ilo = match_lowp(gpu, atoms, params, sig, natoms, sm.MRF_MATCH_BF16_TOP8, b1map=b1)
+    assert np.mean(ilo == i32) >= 0.999
+    assert np.all(ilo // natoms == ks)
+
+
+def test_split_bf16_reference_kat(gpu, golden):
+    """The near-degenerate matching fixture (max off-diagonal |<a_i,a_j>| =
+    0.9999941) through the split-BF16 path: exact indices."""
+    atoms = ra_util.read(f"{golden}/atoms_mrf.ra")
+    params = ra_util.read(f"{golden}/params_mrf.ra").copy()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    idx = [libc.rand() % 109 for _ in range(10)]
+    maps, got = match_lowp(gpu, atoms, params, atoms[idx].copy(), 109, sm.MRF_MATCH_SPLIT3)
+    assert got.tolist() == idx
+    # plain BF16 cannot rank this fixture: more than 8 atoms lie within BF16
+    # rounding (~4e-4 relative at T = 100) of each voxel's own atom, so the
+    # top-8 may miss it; its pick is still one of those near-duplicates
+    sig = atoms[idx].copy()
+    maps, got16 = match_lowp(gpu, atoms, params, sig, 109, sm.MRF_MATCH_BF16_TOP8)
+    assert np.all(fp64_score(atoms, sig, got16) >= 0.999)
+
+
+@pytest.mark.parametrize("T,V,A", [(37, 5, 300), (16, 130, 1000), (1000, 257, 700)])
+def test_lowp_ragged(gpu, T, V, A):
+    rng = np.random.default_rng(T + V)
+    atoms = (rng.normal(size=(A, T)) + 1j * rng.normal(size=(A, T))).astype(np.complex64)
+    atoms /= np.linalg.norm(atoms, axis=1, keepdims=True)
+    params = np.stack([np.arange(A), np.zeros(A), np.zeros(A), np.ones(A)], 1).astype(np.float32)
+    sig, truth, pd = synth.signals_np(atoms, V, seed=5, snr=8.0)
+    for mode in (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_SPLIT3):
+        maps, idx = match_lowp(gpu, atoms, params, sig, A, mode)
+        _check_against_oracle(maps, idx, atoms, params, sig)
