@@ -108,7 +108,7 @@ def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_
     def timed1(fn, k):
         fn()
         torch.cuda.synchronize()
-        return 0.0
+        return float("nan")   # rates are not reported in profile mode
     precision_form(sm, torch, np, synth, raw, T, w, stream, 1, timed1, 0)
     log("profile: precision form done")
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
@@ -133,8 +133,9 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     d_params = torch.from_numpy(params).cuda()
     d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
     sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms, stream=stream)
-    d_hi = torch.empty((A, T), dtype=torch.int32, device="cuda")
-    d_lo = torch.empty((A, T), dtype=torch.int32, device="cuda")
+    ld = sm.lowp_ld(T)
+    d_hi = torch.empty((A, ld), dtype=torch.int32, device="cuda")
+    d_lo = torch.empty((A, ld), dtype=torch.int32, device="cuda")
     prep_ms = timed(lambda: sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo, stream=stream), 1)
     sigs, b1s, truths = [], [], []
     for sl in range(8):
@@ -150,10 +151,13 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     V = d_sig.shape[0]
     res = {"workload": f"epg_ssfp {A} atoms (B0 -40:4:40) x T={T}, 240x240x8 = {V} voxels, "
                        f"B1-map slice selection", "atoms": A, "voxels": V,
-           "dictionary_bytes": {"c64": A * T * 8, "bf16": A * T * 4, "split_bf16": A * T * 8},
+           "dictionary_bytes": {"c64": A * T * 8, "bf16": A * ld * 4, "split_bf16": A * ld * 8},
            "lowp_prepare_ms": prep_ms}
     out = {}
-    for name, mode, k in (("fp32", sm.MRF_MATCH_FP32, 1), ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
+    for name, mode, k in (("fp32", sm.MRF_MATCH_FP32, 1),
+                          ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
+                          ("bf16_top16", sm.MRF_MATCH_BF16_TOP16, steps),
+                          ("split2", sm.MRF_MATCH_SPLIT2, steps),
                           ("split3", sm.MRF_MATCH_SPLIT3, steps)):
         d_maps = torch.empty((V, 5), device="cuda")
         d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
@@ -171,14 +175,18 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                      "truth_agree": float(np.mean(out[name][1] == truth))}
         log(f"precision form {name}: {ms:.1f} ms/call")
     i32 = out["fp32"][1]
-    for name in ("bf16_top8", "split3"):
+    lowp_names = ("bf16_top8", "bf16_top16", "split2", "split3")
+    for name in lowp_names:
         res[name]["argmax_agree_fp32"] = float(np.mean(out[name][1] == i32))
         res[name]["speedup_vs_fp32"] = res[name]["voxels_per_s"] / res["fp32"]["voxels_per_s"]
-    res["roofline_bf16"] = {"bound": "mfma", "achieved": res["bf16_top8"]["tflops_algorithmic"],
+    # MFMA work of the split forms is 2x / 3x the algorithmic flops
+    for name, nprod in (("split2", 2), ("split3", 3)):
+        res[name]["tflops_mfma"] = res[name]["tflops_algorithmic"] * nprod
+    res["roofline_bf16"] = {"bound": "mfma", "achieved": res["bf16_top16"]["tflops_algorithmic"],
                             "peak": PEAK_BF16, "unit": "TFLOP/s",
-                            "frac": res["bf16_top8"]["tflops_algorithmic"] / PEAK_BF16,
+                            "frac": res["bf16_top16"]["tflops_algorithmic"] / PEAK_BF16,
                             "traffic": measured_traffic("match_bf16"),
-                            "kernel": "match_lowp_kernel<false,32> + rescore_kernel"}
+                            "kernel": "match_lowp_kernel<1> + rescore_kernel<16>"}
     # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path
     if oracle_voxels:
         import oracle
@@ -199,7 +207,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
             ref[sel, 4] = rpd
         log(f"FP64 oracle on {oracle_voxels} voxels: {time.time() - t0:.1f} s")
         errs = {}
-        for name in ("fp32", "bf16_top8", "split3"):
+        for name in ("fp32",) + lowp_names:
             m = out[name][0][sub]
             errs[name] = {"T1_mae_ms": float(np.mean(np.abs(m[:, 0] - ref[:, 0]))),
                           "T2_mae_ms": float(np.mean(np.abs(m[:, 1] - ref[:, 1]))),

@@ -147,12 +147,15 @@ int mrf_match(float *d_maps, const mrf_complex *h_img,
 /* Matching precisions of mrf_match_ex */
 #define MRF_MATCH_FP32 0        /* complex64 dictionary, FP32 MFMA */
 #define MRF_MATCH_BF16_TOP8 1   /* BF16 MFMA + top-8 + FP32 rescore */
-#define MRF_MATCH_SPLIT3 2      /* split-BF16 3-pass + top-8 + FP32 rescore */
+#define MRF_MATCH_SPLIT3 2      /* split-BF16 3 products (hi*hi + lo*hi + hi*lo) + top-8 + rescore */
+#define MRF_MATCH_BF16_TOP16 3  /* BF16 MFMA + top-16 + FP32 rescore */
+#define MRF_MATCH_SPLIT2 4      /* dictionary split (a_hi + a_lo) * s_hi + top-8 + rescore */
 
 /* Extended matching, all device-resident, on a caller stream.
  *   d_atoms   [NATOMS][T] complex64 (used by FP32 and for the rescore)
- *   d_lowp    [NATOMS][T] bf16x2 hi copy (BF16/SPLIT3 modes; from
- *             mrf_lowp_prepare), d_lowp_lo the lo residual (SPLIT3)
+ *   d_lowp    [NATOMS][mrf_lowp_ld(T)] bf16x2 hi copy (all low-precision
+ *             modes; from mrf_lowp_prepare), d_lowp_lo the lo residual
+ *             (SPLIT2, SPLIT3)
  *   d_sig     [V][T] complex64 signals
  *   d_b1map   [V] float B1 map or NULL (max-PD merge over all slices)
  *   d_maps    [V][5] (T1, T2, B0, B1, PD); d_idx optional [V] int32 global
@@ -168,9 +171,16 @@ int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                  const float *d_b1map, float *d_maps, int32_t *d_idx);
 
 /* Builds the BF16 copies of a complex64 dictionary: d_hi = bf16(a),
- * d_lo = bf16(a - hi) (optional).  Each is [NATOMS][T] x 2 bf16. */
+ * d_lo = bf16(a - hi) (optional), round to nearest even.  Each is
+ * [NATOMS][mrf_lowp_ld(T)] words of two bf16 (re in the low half, im in the
+ * high half), the pad past T zero.  No reference counterpart (the reference
+ * matches in FP32 only, src/functions.cu:1386-1426). */
 int mrf_lowp_prepare(void *stream, const mrf_complex *d_atoms, size_t T,
                      size_t NATOMS, void *d_hi, void *d_lo);
+
+/* Row stride in 32-bit words of the mrf_lowp_prepare copies: T rounded up to
+ * a multiple of 32. */
+size_t mrf_lowp_ld(size_t T);
 
 /* Device count visible to the library (0 when no GPU). */
 int mrf_device_count(void);

@@ -20,6 +20,8 @@ CLI_PATH = os.path.join(_HERE, "bin", "mrf")
 MRF_MATCH_FP32 = 0
 MRF_MATCH_BF16_TOP8 = 1
 MRF_MATCH_SPLIT3 = 2
+MRF_MATCH_BF16_TOP16 = 3
+MRF_MATCH_SPLIT2 = 4
 
 ECHO_TYPES = ("epg_ssfp", "roa_ssfp", "epg_sr", "epg_se", "epg_tse1", "epg_tse2", "epg_fisp")
 
@@ -28,7 +30,7 @@ EXPORTS = (
     "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
     "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
     "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
-    "mrf_match_ex", "mrf_lowp_prepare", "mrf_device_count",
+    "mrf_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_device_count",
 )
 
 
@@ -81,6 +83,7 @@ def lib() -> ctypes.CDLL:
         "mrf_match": ([P, P, P, P, S, S, S, S, S, I, I, cs], I),
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
         "mrf_lowp_prepare": ([P, P, S, S, P, P], I),
+        "mrf_lowp_ld": ([S], S),
         "mrf_device_count": ([], I),
     }
     for name, (args, res) in sig.items():
@@ -272,6 +275,12 @@ def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_s
                              _ptr(d_b1map), _ptr(d_maps), _ptr(d_idx)))
 
 
+def lowp_ld(T: int) -> int:
+    """Row stride (32-bit words) of the BF16 dictionary copies."""
+    return int(lib().mrf_lowp_ld(T))
+
+
 def lowp_prepare(d_atoms, T: int, NATOMS: int, d_hi, d_lo=None, stream=None) -> None:
+    """d_hi/d_lo: device buffers of NATOMS * lowp_ld(T) 32-bit words."""
     check(lib().mrf_lowp_prepare(_stream_ptr(stream), _ptr(d_atoms), T, NATOMS, _ptr(d_hi),
                                  _ptr(d_lo)))

@@ -220,7 +220,7 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                         NATOMS, natoms);
     if (mode != MRF_MATCH_FP32 && !d_lowp)
         return mrf_fail(MRF_E_ARG, "low-precision mode needs the bf16 dictionary");
-    if (mode == MRF_MATCH_SPLIT3 && !d_lowp_lo)
+    if ((mode == MRF_MATCH_SPLIT3 || mode == MRF_MATCH_SPLIT2) && !d_lowp_lo)
         return mrf_fail(MRF_E_ARG, "split mode needs the bf16 lo dictionary");
     if (int rc = check_device()) return rc;
     if (V == 0) return MRF_OK;
@@ -308,7 +308,15 @@ extern "C" int mrf_lowp_prepare(void *stream, const mrf_complex *d_atoms, size_t
 {
     mrf_clear_error();
     if (!d_atoms || !d_hi) return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (T == 0 || T > (size_t)kLowpMaxT)
+        return mrf_fail(MRF_E_ARG, "low-precision copies support 1 <= T <= %d", kLowpMaxT);
     if (int rc = check_device()) return rc;
-    return launch_lowp_prepare(reinterpret_cast<const float2 *>(d_atoms), NATOMS, (int)T,
-                               (uint32_t *)d_hi, (uint32_t *)d_lo, (hipStream_t)stream);
+    return launch_lowp_prepare(reinterpret_cast<const float2 *>(d_atoms), nullptr, NATOMS, (int)T,
+                               (int)lowp_ld(T), (uint32_t *)d_hi, (uint32_t *)d_lo,
+                               (hipStream_t)stream);
+}
+
+extern "C" size_t mrf_lowp_ld(size_t T)
+{
+    return lowp_ld(T);
 }

@@ -49,8 +49,8 @@ int launch_normalize(float2 *atoms, const double *norm2, size_t n, int T, hipStr
 
 struct MatchLaunch {
     const float2 *atoms;        // [NATOMS][T]
-    const uint32_t *lowp_hi;    // [NATOMS][T] bf16x2 or nullptr
-    const uint32_t *lowp_lo;    // [NATOMS][T] bf16x2 or nullptr
+    const uint32_t *lowp_hi;    // [NATOMS][lowp_ld(T)] bf16x2 or nullptr
+    const uint32_t *lowp_lo;    // [NATOMS][lowp_ld(T)] bf16x2 or nullptr
     const float *params;        // [NATOMS][4]
     const float2 *sig;          // [V][T]
     const int32_t *vox_slice;   // [V] slice index per voxel, or nullptr = all slices
@@ -66,8 +66,15 @@ struct MatchLaunch {
 size_t match_scratch_bytes(const MatchLaunch &a);
 int launch_match(const MatchLaunch &a, void *scratch, hipStream_t s);
 
-int launch_lowp_prepare(const float2 *atoms, size_t n, int T, uint32_t *hi, uint32_t *lo,
-                        hipStream_t s);
+// Row stride (bf16x2 words) of the low-precision copies: T rounded up to a
+// multiple of 32, the pad zero-filled.
+constexpr int kLowpMaxT = 1 << 20;
+inline size_t lowp_ld(size_t T) { return (T + 31) / 32 * 32; }
+
+// hi[r][t] = bf16(src[perm ? perm[r] : r][t]), lo = bf16(src - hi) (optional),
+// rows of lowp_ld(T) words, t >= T zero.
+int launch_lowp_prepare(const float2 *src, const int32_t *perm, size_t rows, int T, int ld,
+                        uint32_t *hi, uint32_t *lo, hipStream_t s);
 
 // B1 slice per voxel: argmin_k |b1map[v] - slice_b1[k]| (strict <, init 10).
 int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,

@@ -1,6 +1,7 @@
-// lowp.hip -- BF16 copies of a complex64 dictionary for the low-precision
-// matcher: hi = bf16(x) (round to nearest even), lo = bf16(x - hi).  Each
-// complex value becomes one 32-bit word (re in the low half, im in the high).
+// lowp.hip -- BF16 copies of complex64 rows (dictionary atoms or signals) for
+// the low-precision matcher: hi = bf16(x) (round to nearest even),
+// lo = bf16(x - hi).  Each complex value becomes one 32-bit word (re in the
+// low half, im in the high); rows are padded with zero words to the stride.
 #include "kernels.h"
 #include "snapmrf.h"
 
@@ -19,31 +20,37 @@ __device__ __forceinline__ float bf16_to_f(uint32_t h) { return __uint_as_float(
 
 namespace {
 
-__global__ void lowp_kernel(const float2 *atoms, size_t n, uint32_t *hi, uint32_t *lo)
+// One work-group per row (grid-stride over rows), one word per thread.
+__global__ void lowp_kernel(const float2 *src, const int32_t *perm, size_t rows, int T, int ld,
+                            uint32_t *hi, uint32_t *lo)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const float2 v = atoms[i];
-        const uint32_t hr = bf16_rne(v.x), hm = bf16_rne(v.y);
-        hi[i] = hr | (hm << 16);
-        if (lo) {
-            const uint32_t lr = bf16_rne(v.x - bf16_to_f(hr));
-            const uint32_t lm = bf16_rne(v.y - bf16_to_f(hm));
-            lo[i] = lr | (lm << 16);
+    for (size_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const float2 *row = src + (size_t)(perm ? perm[r] : (int32_t)r) * T;
+        uint32_t *h = hi + r * ld;
+        uint32_t *l = lo ? lo + r * ld : nullptr;
+        for (int t = threadIdx.x; t < ld; t += blockDim.x) {
+            uint32_t hw = 0u, lw = 0u;
+            if (t < T) {
+                const float2 v = row[t];
+                const uint32_t hr = bf16_rne(v.x), hm = bf16_rne(v.y);
+                hw = hr | (hm << 16);
+                lw = bf16_rne(v.x - bf16_to_f(hr)) | (bf16_rne(v.y - bf16_to_f(hm)) << 16);
+            }
+            h[t] = hw;
+            if (l) l[t] = lw;
         }
     }
 }
 
 }  // namespace
 
-int launch_lowp_prepare(const float2 *atoms, size_t n, int T, uint32_t *hi, uint32_t *lo,
-                        hipStream_t s)
+int launch_lowp_prepare(const float2 *src, const int32_t *perm, size_t rows, int T, int ld,
+                        uint32_t *hi, uint32_t *lo, hipStream_t s)
 {
-    const size_t total = n * (size_t)T;
-    if (total == 0) return 0;
-    size_t blocks = (total + 255) / 256;
-    if (blocks > 65536 * 4) blocks = 65536 * 4;
-    lowp_kernel<<<dim3((unsigned)blocks), 256, 0, s>>>(atoms, total, hi, lo);
+    if (rows == 0 || T == 0) return 0;
+    if (ld < T || (ld & 31)) return mrf_fail(MRF_E_ARG, "lowp: bad row stride %d for T=%d", ld, T);
+    const size_t blocks = rows < 65536 ? rows : 65536;
+    lowp_kernel<<<dim3((unsigned)blocks), 256, 0, s>>>(src, perm, rows, T, ld, hi, lo);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "lowp launch: %s", hipGetErrorString(e));
 }

@@ -284,7 +284,7 @@ __global__ void count_kernel(const int32_t *vox_slice, size_t V, int n_slices, i
 }
 
 // single work-item: exclusive scan of the slice counts and the tile list
-__global__ void tiles_kernel(const int *counts, int n_slices, int *offsets, Tile *tiles,
+__global__ void tiles_kernel(const int *counts, int n_slices, int vt, int *offsets, Tile *tiles,
                              int *ntiles)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -292,11 +292,11 @@ __global__ void tiles_kernel(const int *counts, int n_slices, int *offsets, Tile
     for (int k = 0; k < n_slices; k++) {
         offsets[k] = off;
         const int c = counts[k];
-        for (int p = 0; p < c; p += VT) {
+        for (int p = 0; p < c; p += vt) {
             Tile t;
             t.slice = k;
             t.pstart = off + p;
-            t.count = (c - p < VT) ? c - p : VT;
+            t.count = (c - p < vt) ? c - p : vt;
             t.pad = 0;
             tiles[nt++] = t;
         }
@@ -362,30 +362,64 @@ __global__ void zero_i32_kernel(int *p, size_t n)
 }
 
 // ------------------------------------------------------------------------
-// BF16 matcher: v_mfma_f32_16x16x32_bf16 over BF16 copies of the dictionary
-// and the signals, FP32 accumulation, a per-voxel top-8 candidate list and an
-// FP32 rescore of the candidates from the complex64 dictionary.  SPLIT adds
-// the hi*lo and lo*hi products of the split-BF16 representation
-// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)).
+// Low-precision matchers: v_mfma_f32_16x16x32_bf16 over BF16 copies of the
+// dictionary and the signals with FP32 accumulation, candidate lists per
+// voxel, and an FP32 rescore of the final candidates from the complex64
+// dictionary.  NPROD selects the products of the split representation
+// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)):
+//   1: a_hi*s_hi                       (BF16)
+//   2: a_hi*s_hi + a_lo*s_hi           (dictionary split; the signal's own
+//                                       rounding shifts every atom's score
+//                                       together, so the ranking is kept)
+//   3: a_hi*s_hi + a_lo*s_hi + a_hi*s_lo (split-BF16, 3 products)
+//
+// Operands are bf16x2 words (re low, im high), rows padded with zeros to a
+// stride `ld` (a multiple of 32 words); the signals are converted in tile
+// order, so a tile's voxels are contiguous rows.  Rows are fetched with
+// buffer loads whose range check returns zeros for rows past the end of a
+// sub-tile or tile, so the staging needs no per-lane guards.
+//
+// Work-group tile: 128 atoms x 256 voxels, 4 waves of 128 atoms x 64 voxels
+// (acc 8 x 4 MFMA blocks, Re and Im); Y = (xi, -xr) is formed in registers
+// from the X fragment.  K chunks of 16 timepoints are double-buffered in
+// LDS (one barrier per chunk).  Work-groups are numbered so that the ones on
+// one XCD (work-group id mod 8) take consecutive atom chunks of the same
+// voxel tile, which then streams from that XCD's L2.
+//
+// Candidates: MFMA row 4g + r of block i holds atom sub + g + 4r + 16i, so
+// lane group g sees the atoms = g (mod 4) and neighbouring atoms (adjacent
+// T1 grid points) fall to different lanes.  Each lane keeps a sorted top-4
+// per voxel; the 4 lists of a voxel form its 16 candidates per atom chunk;
+// the rescore kernel merges the chunks, keeps the top K and rescores them in
+// FP32 (first index on ties, as generate_maps, src/functions.cu:1317-1349).
 // ------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTopK = 8;
-constexpr int kMaxChunks = 32;
+constexpr int LAT = 128;          // atoms per work-group sub-tile
+constexpr int LVT = 256;          // voxels per work-group tile (64 per wave)
+constexpr int LKC = 16;           // timepoints per K chunk
+constexpr int LSB = LKC + 4;      // dwords per LDS row (padded)
+constexpr int kLaneK = 4;         // candidates per lane and voxel
+constexpr int kChunkK = 16;       // candidates per voxel and atom chunk (4 lanes)
+constexpr int kMaxChunks = 32;    // rescore merges up to 512 entries per voxel
+constexpr int kMinChunkAtoms = 4 * LAT;
+constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
 
 struct LowpArgs {
-    const uint32_t *ahi, *alo;   // [NATOMS][T] bf16x2 (re low half, im high half)
-    const uint32_t *shi, *slo;   // [V][T] bf16x2, original voxel order
+    const uint32_t *ahi, *alo;   // [NATOMS][ld] bf16x2
+    const uint32_t *shi, *slo;   // [V][ld] bf16x2 in tile (perm) order
     const int32_t *perm;
     const Tile *tiles;
     const int *ntiles;
-    float *part_s;               // [chunks][V][kTopK] candidate scores (-1 = empty)
-    uint32_t *part_i;            // [chunks][V][kTopK] global atom indices
+    float *part_s;               // [chunks][V][kChunkK] scores (-1 = empty)
+    uint32_t *part_i;            // [chunks][V][kChunkK] global atom indices
     size_t natoms;               // atoms per slice
     size_t chunk;                // atoms per work-group
     size_t V;
-    int T;
+    int ld;                      // row stride (words)
+    int nk;                      // K chunks, ceil(T / LKC)
+    int nch;                     // atom chunks per tile
 };
 
 // (re, im) -> (im, -re) on a bf16x2 word: the Y operand of Im S.
@@ -394,17 +428,45 @@ __device__ __forceinline__ uint32_t swap_neg(uint32_t w)
     return ((w >> 16) | (w << 16)) ^ 0x80000000u;
 }
 
-// Four consecutive bf16x2 words (timepoints t..t+3) of a row, zero outside
-// [0, T) or for a null row; rows are 16-byte aligned when T % 4 == 0.
-__device__ __forceinline__ u32x4 load4(const uint32_t *row, int t, int T, bool al4)
+__device__ __forceinline__ u32x4 swap_neg4(u32x4 x)
 {
-    u32x4 v = u32x4{0u, 0u, 0u, 0u};
-    if (!row) return v;
-    if (al4 && t + 3 < T) return *reinterpret_cast<const u32x4 *>(row + t);
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-        if (t + u < T) v[u] = row[t + u];
-    return v;
+    return u32x4{swap_neg(x[0]), swap_neg(x[1]), swap_neg(x[2]), swap_neg(x[3])};
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const uint32_t *p, size_t bytes)
+{
+    const uint32_t n = bytes > 0xffffffffu ? 0xffffffffu : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(p), (short)0, (int)n,
+                                             (int)kRsrcWord3);
+}
+
+// 16 bytes at byte offset voff + soff; zeros when voff is past the resource
+__device__ __forceinline__ u32x4 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+}
+
+// Insert (sc, gi) into the sorted 4-entry list at ls/li (score descending;
+// equal scores keep the earlier, lower index first), given that sc beats the
+// last entry.  Branch-free; returns the new last score.
+__device__ __forceinline__ float lane_insert(float *ls, uint32_t *li, float sc, uint32_t gi)
+{
+    const float4 S = *reinterpret_cast<const float4 *>(ls);
+    const uint4 I = *reinterpret_cast<const uint4 *>(li);
+    const bool b0 = S.x >= sc, b1 = S.y >= sc, b2 = S.z >= sc;
+    float4 So;
+    uint4 Io;
+    So.x = b0 ? S.x : sc;
+    Io.x = b0 ? I.x : gi;
+    So.y = b1 ? S.y : (b0 ? sc : S.x);
+    Io.y = b1 ? I.y : (b0 ? gi : I.x);
+    So.z = b2 ? S.z : (b1 ? sc : S.y);
+    Io.z = b2 ? I.z : (b1 ? gi : I.y);
+    So.w = b2 ? sc : S.z;
+    Io.w = b2 ? gi : I.z;
+    *reinterpret_cast<float4 *>(ls) = So;
+    *reinterpret_cast<uint4 *>(li) = Io;
+    return So.w;
 }
 
 __device__ __forceinline__ floatx4 mfma_bf16(u32x4 a, u32x4 b, floatx4 c)
@@ -413,218 +475,202 @@ __device__ __forceinline__ floatx4 mfma_bf16(u32x4 a, u32x4 b, floatx4 c)
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-template <bool SPLIT, int KC>
+// atom offset within the sub-tile held by LDS row s = 16i + 4g + r
+__device__ __forceinline__ int row_atom(int s)
+{
+    return ((s >> 2) & 3) + 4 * (s & 3) + (s & ~15);
+}
+
+template <int NPROD>
 __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
 {
-    constexpr int LSB = KC + 4;             // dwords per LDS row (padded)
-    constexpr int SEGS = KC / 4;            // 16-byte segments per row chunk
+    constexpr bool ASPLIT = NPROD >= 2;
+    constexpr bool SSPLIT = NPROD >= 3;
+    constexpr int SEGS = LKC / 4;           // 16-byte segments per row chunk
     constexpr int RPP = kBlock / SEGS;      // rows staged per pass
-    constexpr int PASSES = AT / RPP;
-    constexpr int KSTEPS = KC / 16;         // 16 complex = 32 bf16 per MFMA
-    static_assert(AT == VT, "square work-group tile");
+    constexpr int PA = LAT / RPP;
+    constexpr int PX = LVT / RPP;
+    constexpr int BA = LAT * LSB, BX = LVT * LSB;
 
-    const int tile_id = blockIdx.x;
+    // XCD-aware numbering: work-group L runs on XCD L % 8
+    unsigned L = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) L = (L & 7u) * (gridDim.x >> 3) + (L >> 3);
+    const int tile_id = (int)(L / (unsigned)a.nch);
+    const int chunk_id = (int)(L % (unsigned)a.nch);
     if (tile_id >= *a.ntiles) return;
     const Tile tile = a.tiles[tile_id];
-    const size_t a_begin = (size_t)blockIdx.y * a.chunk;
+    const size_t a_begin = (size_t)chunk_id * a.chunk;
     if (a_begin >= a.natoms) return;
     const size_t a_end = (a_begin + a.chunk < a.natoms) ? a_begin + a.chunk : a.natoms;
     const size_t slice_base = (size_t)tile.slice * a.natoms;
-    const bool al4 = (a.T & 3) == 0;
+    const size_t ldb = (size_t)a.ld * 4;    // row stride in bytes
 
-    __shared__ __attribute__((aligned(16))) uint32_t lA[AT * LSB];
-    __shared__ __attribute__((aligned(16))) uint32_t lX[VT * LSB];
-    __shared__ __attribute__((aligned(16))) uint32_t lY[VT * LSB];
-    __shared__ __attribute__((aligned(16))) uint32_t lAl[SPLIT ? AT * LSB : 4];
-    __shared__ __attribute__((aligned(16))) uint32_t lXl[SPLIT ? VT * LSB : 4];
-    __shared__ __attribute__((aligned(16))) uint32_t lYl[SPLIT ? VT * LSB : 4];
-    __shared__ float cs[2 * 4 * VT * kTopK];
-    __shared__ uint32_t ci[2 * 4 * VT * kTopK];
+    __shared__ __attribute__((aligned(16))) uint32_t lA[2 * BA];
+    __shared__ __attribute__((aligned(16))) uint32_t lAl[ASPLIT ? 2 * BA : 4];
+    __shared__ __attribute__((aligned(16))) uint32_t lX[2 * BX];
+    __shared__ __attribute__((aligned(16))) uint32_t lXl[SSPLIT ? 2 * BX : 4];
+    __shared__ __attribute__((aligned(16))) float cs[4 * LVT * kLaneK];
+    __shared__ __attribute__((aligned(16))) uint32_t ci[4 * LVT * kLaneK];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wa = wave & 1;
-    const int wv = wave >> 1;
+    const int w = tid >> 6;                 // 64-voxel quarter of the tile
     const int g = lane >> 4;
     const int r16 = lane & 15;
     const int st_row = tid / SEGS;
-    const int st_t = (tid % SEGS) * 4;      // timepoint offset within the chunk
+    const uint32_t st_b = (uint32_t)(tid % SEGS) * 16u;
 
-    for (int q = tid; q < 2 * 4 * VT * kTopK; q += kBlock) {
+    for (int q = tid; q < 4 * LVT * kLaneK; q += kBlock) {
         cs[q] = -1.0f;
         ci[q] = 0xffffffffu;
     }
-    float th[4] = {-1.0f, -1.0f, -1.0f, -1.0f};   // 8th best of this lane's lists
+    float th[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
 
-    const uint32_t *vh[PASSES], *vl[PASSES];
+    const size_t xrow0 = (size_t)tile.pstart * a.ld;
+    const __amdgpu_buffer_rsrc_t rX = make_rsrc(a.shi + xrow0, (size_t)tile.count * ldb);
+    __amdgpu_buffer_rsrc_t rXl = rX;
+    if (SSPLIT) rXl = make_rsrc(a.slo + xrow0, (size_t)tile.count * ldb);
+    uint32_t xoff[PX], aoff[PA];
 #pragma unroll
-    for (int p = 0; p < PASSES; p++) {
-        const int vr = st_row + RPP * p;
-        const size_t off = (vr < tile.count) ? (size_t)a.perm[tile.pstart + vr] * (size_t)a.T : 0;
-        vh[p] = (vr < tile.count) ? a.shi + off : nullptr;
-        vl[p] = (SPLIT && vr < tile.count) ? a.slo + off : nullptr;
-    }
-    const int nchunks = (a.T + KC - 1) / KC;
+    for (int p = 0; p < PX; p++) xoff[p] = (uint32_t)((st_row + RPP * p) * ldb) + st_b;
+#pragma unroll
+    for (int p = 0; p < PA; p++) aoff[p] = (uint32_t)(row_atom(st_row + RPP * p) * ldb) + st_b;
 
-    for (size_t sub = a_begin; sub < a_end; sub += AT) {
-        const uint32_t *ah[PASSES], *al[PASSES];
+    for (size_t sub = a_begin; sub < a_end; sub += LAT) {
+        const size_t nsub = (a_end - sub < (size_t)LAT) ? a_end - sub : (size_t)LAT;
+        const size_t arow0 = (slice_base + sub) * a.ld;
+        const __amdgpu_buffer_rsrc_t rA = make_rsrc(a.ahi + arow0, nsub * ldb);
+        __amdgpu_buffer_rsrc_t rAl = rA;
+        if (ASPLIT) rAl = make_rsrc(a.alo + arow0, nsub * ldb);
+
+        floatx4 acc_re[8][4], acc_im[8][4];
 #pragma unroll
-        for (int p = 0; p < PASSES; p++) {
-            const size_t ar = sub + st_row + RPP * p;
-            const size_t off = (slice_base + ar) * (size_t)a.T;
-            ah[p] = (ar < a_end) ? a.ahi + off : nullptr;
-            al[p] = (SPLIT && ar < a_end) ? a.alo + off : nullptr;
-        }
-        floatx4 acc_re[4][4], acc_im[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 8; i++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
                 acc_im[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
-        u32x4 ga[PASSES], gx[PASSES], gal[PASSES], gxl[PASSES];
+        u32x4 ga[PA], gal[PA], gx[PX], gxl[PX];
+        auto stage_load = [&](int c) {
+            const uint32_t so = (uint32_t)c * (LKC * 4);
 #pragma unroll
-        for (int p = 0; p < PASSES; p++) {
-            ga[p] = load4(ah[p], st_t, a.T, al4);
-            gx[p] = load4(vh[p], st_t, a.T, al4);
-            if (SPLIT) {
-                gal[p] = load4(al[p], st_t, a.T, al4);
-                gxl[p] = load4(vl[p], st_t, a.T, al4);
+            for (int p = 0; p < PA; p++) {
+                ga[p] = bload(rA, aoff[p], so);
+                if (ASPLIT) gal[p] = bload(rAl, aoff[p], so);
             }
-        }
-        for (int c = 0; c < nchunks; c++) {
-            __syncthreads();
 #pragma unroll
-            for (int p = 0; p < PASSES; p++) {
-                const int o = (st_row + RPP * p) * LSB + st_t;
+            for (int p = 0; p < PX; p++) {
+                gx[p] = bload(rX, xoff[p], so);
+                if (SSPLIT) gxl[p] = bload(rXl, xoff[p], so);
+            }
+        };
+        auto stage_store = [&](int b) {
+#pragma unroll
+            for (int p = 0; p < PA; p++) {
+                const int o = b * BA + (st_row + RPP * p) * LSB + (int)(st_b >> 2);
                 *reinterpret_cast<u32x4 *>(&lA[o]) = ga[p];
+                if (ASPLIT) *reinterpret_cast<u32x4 *>(&lAl[o]) = gal[p];
+            }
+#pragma unroll
+            for (int p = 0; p < PX; p++) {
+                const int o = b * BX + (st_row + RPP * p) * LSB + (int)(st_b >> 2);
                 *reinterpret_cast<u32x4 *>(&lX[o]) = gx[p];
-                *reinterpret_cast<u32x4 *>(&lY[o]) =
-                    u32x4{swap_neg(gx[p][0]), swap_neg(gx[p][1]), swap_neg(gx[p][2]),
-                          swap_neg(gx[p][3])};
-                if (SPLIT) {
-                    *reinterpret_cast<u32x4 *>(&lAl[o]) = gal[p];
-                    *reinterpret_cast<u32x4 *>(&lXl[o]) = gxl[p];
-                    *reinterpret_cast<u32x4 *>(&lYl[o]) =
-                        u32x4{swap_neg(gxl[p][0]), swap_neg(gxl[p][1]), swap_neg(gxl[p][2]),
-                              swap_neg(gxl[p][3])};
-                }
+                if (SSPLIT) *reinterpret_cast<u32x4 *>(&lXl[o]) = gxl[p];
             }
-            __syncthreads();
-            if (c + 1 < nchunks) {
-                const int tc = (c + 1) * KC + st_t;
+        };
+        stage_load(0);
+        stage_store(0);
+        __syncthreads();
+        for (int c = 0; c < a.nk; c++) {
+            const int b = c & 1;
+            if (c + 1 < a.nk) stage_load(c + 1);
+            u32x4 fx[4], fy[4], fxl[4], fyl[4];
 #pragma unroll
-                for (int p = 0; p < PASSES; p++) {
-                    ga[p] = load4(ah[p], tc, a.T, al4);
-                    gx[p] = load4(vh[p], tc, a.T, al4);
-                    if (SPLIT) {
-                        gal[p] = load4(al[p], tc, a.T, al4);
-                        gxl[p] = load4(vl[p], tc, a.T, al4);
-                    }
+            for (int v = 0; v < 4; v++) {
+                const int o = b * BX + (w * 64 + v * 16 + r16) * LSB + g * 4;
+                fx[v] = *reinterpret_cast<const u32x4 *>(&lX[o]);
+                fy[v] = swap_neg4(fx[v]);
+                if (SSPLIT) {
+                    fxl[v] = *reinterpret_cast<const u32x4 *>(&lXl[o]);
+                    fyl[v] = swap_neg4(fxl[v]);
                 }
             }
 #pragma unroll
-            for (int ks = 0; ks < KSTEPS; ks++) {
-                const int ko = ks * 16 + g * 4;
-                u32x4 fa[4], fx[4], fy[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    fa[i] = *reinterpret_cast<const u32x4 *>(&lA[(wa * 64 + i * 16 + r16) * LSB + ko]);
+            for (int i = 0; i < 8; i++) {
+                const int o = b * BA + (i * 16 + r16) * LSB + g * 4;
+                const u32x4 fa = *reinterpret_cast<const u32x4 *>(&lA[o]);
+                u32x4 fal;
+                if (ASPLIT) fal = *reinterpret_cast<const u32x4 *>(&lAl[o]);
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int o = (wv * 64 + v * 16 + r16) * LSB + ko;
-                    fx[v] = *reinterpret_cast<const u32x4 *>(&lX[o]);
-                    fy[v] = *reinterpret_cast<const u32x4 *>(&lY[o]);
+                    acc_re[i][v] = mfma_bf16(fa, fx[v], acc_re[i][v]);
+                    acc_im[i][v] = mfma_bf16(fa, fy[v], acc_im[i][v]);
                 }
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        acc_re[i][v] = mfma_bf16(fa[i], fx[v], acc_re[i][v]);
-                        acc_im[i][v] = mfma_bf16(fa[i], fy[v], acc_im[i][v]);
-                    }
-                if (SPLIT) {
-                    u32x4 fal[4], fxl[4], fyl[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        fal[i] = *reinterpret_cast<const u32x4 *>(
-                            &lAl[(wa * 64 + i * 16 + r16) * LSB + ko]);
+                if (ASPLIT) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        const int o = (wv * 64 + v * 16 + r16) * LSB + ko;
-                        fxl[v] = *reinterpret_cast<const u32x4 *>(&lXl[o]);
-                        fyl[v] = *reinterpret_cast<const u32x4 *>(&lYl[o]);
+                        acc_re[i][v] = mfma_bf16(fal, fx[v], acc_re[i][v]);
+                        acc_im[i][v] = mfma_bf16(fal, fy[v], acc_im[i][v]);
                     }
+                }
+                if (SSPLIT) {
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            acc_re[i][v] = mfma_bf16(fa[i], fxl[v], acc_re[i][v]);
-                            acc_im[i][v] = mfma_bf16(fa[i], fyl[v], acc_im[i][v]);
-                            acc_re[i][v] = mfma_bf16(fal[i], fx[v], acc_re[i][v]);
-                            acc_im[i][v] = mfma_bf16(fal[i], fy[v], acc_im[i][v]);
-                        }
+                    for (int v = 0; v < 4; v++) {
+                        acc_re[i][v] = mfma_bf16(fa, fxl[v], acc_re[i][v]);
+                        acc_im[i][v] = mfma_bf16(fa, fyl[v], acc_im[i][v]);
+                    }
                 }
             }
+            if (c + 1 < a.nk) stage_store(b ^ 1);
+            __syncthreads();
         }
-        // epilogue: fold |S|^2 of this sub-tile into the lane's top-8 lists
+        // epilogue: fold |S|^2 of the sub-tile into this lane's top-4 lists.
+        // 16 scores at a time are tested against the list threshold; the
+        // (rare) insertions run in a short divergent loop in increasing atom
+        // order.
+        const uint32_t lane_atom0 = (uint32_t)(slice_base + sub) + (uint32_t)g;
+        const int nvalid = (int)nsub;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int voxel = wv * 64 + v * 16 + r16;
+            const int q = (g * LVT + w * 64 + v * 16 + r16) * kLaneK;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int h = 0; h < 2; h++) {
+                float sc[16];
+                uint32_t m = 0;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const size_t atom = sub + wa * 64 + i * 16 + 4 * g + r;
-                    const float re = acc_re[i][v][r], im = acc_im[i][v][r];
-                    const float sc = re * re + im * im;
-                    if (atom < a_end && sc > th[v]) {
-                        float *ls = &cs[((wa * 4 + g) * VT + voxel) * kTopK];
-                        uint32_t *li = &ci[((wa * 4 + g) * VT + voxel) * kTopK];
-                        int pos = kTopK - 1;
-                        while (pos > 0 && ls[pos - 1] < sc) {
-                            ls[pos] = ls[pos - 1];
-                            li[pos] = li[pos - 1];
-                            pos--;
-                        }
-                        ls[pos] = sc;
-                        li[pos] = (uint32_t)(slice_base + atom);
-                        th[v] = ls[kTopK - 1];
+                for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = 4 * h + ii, j = 4 * ii + r;
+                        const float re = acc_re[i][v][r], im = acc_im[i][v][r];
+                        sc[j] = re * re + im * im;
+                        if (g + 4 * r + 16 * i < nvalid && sc[j] > th[v]) m |= 1u << j;
                     }
+                while (m) {
+                    const int j = __builtin_ctz(m);
+                    m &= m - 1;
+                    float s1 = sc[0];
+#pragma unroll
+                    for (int k = 1; k < 16; k++) s1 = (j == k) ? sc[k] : s1;
+                    if (s1 > th[v])
+                        th[v] = lane_insert(&cs[q], &ci[q], s1,
+                                            lane_atom0 + 4u * (j & 3) + 16u * (4 * h + (j >> 2)));
                 }
+            }
         }
     }
     __syncthreads();
-    // merge the 8 sorted lists of each voxel into its top-8 for this chunk
+    // the 4 lane lists of each voxel are its 16 candidates of this chunk
     if (tid < tile.count) {
-        int head[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const size_t vox = (size_t)a.perm[tile.pstart + tid];
-        float *ps = a.part_s + ((size_t)blockIdx.y * a.V + vox) * kTopK;
-        uint32_t *pi = a.part_i + ((size_t)blockIdx.y * a.V + vox) * kTopK;
-        for (int e = 0; e < kTopK; e++) {
-            int bh = -1;
-            float bs = -1.0f;
-            uint32_t bi = 0xffffffffu;
-            for (int h = 0; h < 8; h++) {
-                if (head[h] >= kTopK) continue;
-                const int q = (h * VT + tid) * kTopK + head[h];
-                const float s2 = cs[q];
-                const uint32_t i2 = ci[q];
-                if (s2 > bs || (s2 == bs && s2 >= 0.0f && i2 < bi)) {
-                    bs = s2;
-                    bi = i2;
-                    bh = h;
-                }
-            }
-            if (bh >= 0 && bs >= 0.0f) head[bh]++;
-            else {
-                bs = -1.0f;
-                bi = 0xffffffffu;
-            }
-            ps[e] = bs;
-            pi[e] = bi;
+        float4 *ps = reinterpret_cast<float4 *>(a.part_s + ((size_t)chunk_id * a.V + vox) * kChunkK);
+        uint4 *pi = reinterpret_cast<uint4 *>(a.part_i + ((size_t)chunk_id * a.V + vox) * kChunkK);
+#pragma unroll
+        for (int gg = 0; gg < 4; gg++) {
+            const int q = (gg * LVT + tid) * kLaneK;
+            ps[gg] = *reinterpret_cast<const float4 *>(&cs[q]);
+            pi[gg] = *reinterpret_cast<const uint4 *>(&ci[q]);
         }
     }
 }
@@ -634,40 +680,43 @@ __device__ __forceinline__ bool cand_better(float s1, uint32_t i1, float s2, uin
     return s1 > s2 || (s1 == s2 && i1 < i2);
 }
 
-// Per voxel (one wave): merge the chunk candidate lists into the top-8,
-// rescore them in FP32 from the complex64 dictionary and keep the best
-// (first index on ties).
+// Per voxel (one wave): merge the chunk candidate lists, keep the top K by
+// low-precision score, rescore them in FP32 from the complex64 dictionary and
+// keep the best (first index on ties).
+template <int K>
 __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
                                                          const uint32_t *part_i, int nchunks,
                                                          const float2 *atoms, const float2 *sig,
                                                          const float *params, size_t V, int T,
                                                          float *maps, int32_t *idx)
 {
+    constexpr int PER = kMaxChunks * kChunkK / 64;   // entries per lane
     const size_t vox = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (vox >= V) return;
-    const int n = nchunks * kTopK;   // <= kMaxChunks * kTopK = 256
-    float s[4];
-    uint32_t id[4];
+    const int n = nchunks * kChunkK;
+    float s[PER];
+    uint32_t id[PER];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < PER; q++) {
         const int e = lane + 64 * q;
         s[q] = -1.0f;
         id[q] = 0xffffffffu;
         if (e < n) {
-            const int ch = e / kTopK, k = e % kTopK;
-            const size_t o = ((size_t)ch * V + vox) * kTopK + k;
+            const int ch = e / kChunkK, k = e % kChunkK;
+            const size_t o = ((size_t)ch * V + vox) * kChunkK + k;
             s[q] = part_s[o];
             id[q] = part_i[o];
         }
     }
-    uint32_t cand[kTopK];
+    uint32_t cand[K];
     int nc = 0;
-    for (int r = 0; r < kTopK; r++) {
+#pragma unroll
+    for (int r = 0; r < K; r++) {
         float bs = -1.0f;
         uint32_t bi = 0xffffffffu;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < PER; q++)
             if (s[q] >= 0.0f && cand_better(s[q], id[q], bs, bi)) {
                 bs = s[q];
                 bi = id[q];
@@ -680,21 +729,21 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
                 bi = oi;
             }
         }
-        if (bs < 0.0f) break;
-        cand[nc++] = bi;
+        cand[r] = bi;
+        if (bs >= 0.0f) nc = r + 1;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = 0; q < PER; q++)
             if (id[q] == bi) s[q] = -1.0f;
     }
     // FP32 rescore
-    float re[kTopK], im[kTopK];
+    float re[K], im[K];
 #pragma unroll
-    for (int c = 0; c < kTopK; c++) re[c] = im[c] = 0.0f;
+    for (int c = 0; c < K; c++) re[c] = im[c] = 0.0f;
     const float2 *x = sig + vox * (size_t)T;
     for (int t = lane; t < T; t += 64) {
         const float2 xv = x[t];
 #pragma unroll
-        for (int c = 0; c < kTopK; c++)
+        for (int c = 0; c < K; c++)
             if (c < nc) {
                 const float2 av = atoms[(size_t)cand[c] * T + t];
                 re[c] += av.x * xv.x + av.y * xv.y;
@@ -702,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
             }
     }
 #pragma unroll
-    for (int c = 0; c < kTopK; c++)
+    for (int c = 0; c < K; c++)
         for (int off = 32; off >= 1; off >>= 1) {
             re[c] += __shfl_xor(re[c], off);
             im[c] += __shfl_xor(im[c], off);
@@ -710,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
     float bs = -1.0f;
     uint32_t bi = 0u;
 #pragma unroll
-    for (int c = 0; c < kTopK; c++)
+    for (int c = 0; c < K; c++)
         if (c < nc) {
             const float sc = re[c] * re[c] + im[c] * im[c];
             if (cand_better(sc, cand[c], bs, bi)) {
@@ -749,18 +798,21 @@ struct Scratch {
     size_t bytes;
 };
 
-// Work-group atom range of the low-precision path: enough chunks to give
-// about 2048 work-groups, at most kMaxChunks, a multiple of the sub-tile.
-void lowp_chunks(const MatchLaunch &l, size_t nat, size_t max_tiles, size_t *chunk,
-                 size_t *nchunks)
+// Atom chunks per voxel tile of the low-precision path: at least 8 (so the
+// work-groups of one XCD share a voxel tile), enough for about 2048
+// work-groups, at most kMaxChunks; chunks hold whole sub-tiles and at least
+// kMinChunkAtoms atoms.
+void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
 {
     size_t want = (2048 + max_tiles - 1) / max_tiles;
-    if (want < 1) want = 1;
+    if (want < 8) want = 8;
     if (want > (size_t)kMaxChunks) want = kMaxChunks;
     size_t c = (nat + want - 1) / want;
-    c = (c + AT - 1) / AT * AT;
+    if (c < (size_t)kMinChunkAtoms) c = kMinChunkAtoms;
+    c = (c + LAT - 1) / LAT * LAT;
     *chunk = c;
     *nchunks = (nat + c - 1) / c;
+    if (*nchunks == 0) *nchunks = 1;
 }
 
 Scratch carve(void *base, const MatchLaunch &l)
@@ -768,7 +820,8 @@ Scratch carve(void *base, const MatchLaunch &l)
     Scratch s{};
     const int ns = l.vox_slice ? l.n_slices : 1;
     const size_t nat = l.vox_slice ? l.natoms : l.natoms * (size_t)l.n_slices;
-    s.max_tiles = (l.V + VT - 1) / VT + (size_t)ns;
+    const size_t vt = (l.mode == MRF_MATCH_FP32) ? VT : LVT;
+    s.max_tiles = (l.V + vt - 1) / vt + (size_t)ns;
     uint8_t *p = (uint8_t *)base;
     s.keys = (unsigned long long *)p;
     p += align256(l.V * 8);
@@ -785,17 +838,18 @@ Scratch carve(void *base, const MatchLaunch &l)
     s.tiles = (Tile *)p;
     p += align256(s.max_tiles * sizeof(Tile));
     if (l.mode != MRF_MATCH_FP32) {
-        lowp_chunks(l, nat, s.max_tiles, &s.chunk, &s.nchunks);
+        lowp_chunks(nat, s.max_tiles, &s.chunk, &s.nchunks);
+        const size_t ld = lowp_ld(l.T);
         s.shi = (uint32_t *)p;
-        p += align256(l.V * (size_t)l.T * 4);
+        p += align256(l.V * ld * 4);
         if (l.mode == MRF_MATCH_SPLIT3) {
             s.slo = (uint32_t *)p;
-            p += align256(l.V * (size_t)l.T * 4);
+            p += align256(l.V * ld * 4);
         }
         s.part_s = (float *)p;
-        p += align256(s.nchunks * l.V * kTopK * 4);
+        p += align256(s.nchunks * l.V * kChunkK * 4);
         s.part_i = (uint32_t *)p;
-        p += align256(s.nchunks * l.V * kTopK * 4);
+        p += align256(s.nchunks * l.V * kChunkK * 4);
     }
     s.bytes = (size_t)(p - (uint8_t *)base);
     return s;
@@ -821,10 +875,15 @@ int launch_b1_select(const float *b1map, const float *params, size_t natoms, int
 int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
 {
     if (l.V == 0) return 0;
-    if (l.mode != MRF_MATCH_FP32 && l.mode != MRF_MATCH_BF16_TOP8 && l.mode != MRF_MATCH_SPLIT3)
+    const bool lowp = l.mode != MRF_MATCH_FP32;
+    if (l.mode < MRF_MATCH_FP32 || l.mode > MRF_MATCH_SPLIT2)
         return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", l.mode);
-    if (l.mode != MRF_MATCH_FP32 && (!l.lowp_hi || (l.mode == MRF_MATCH_SPLIT3 && !l.lowp_lo)))
-        return mrf_fail(MRF_E_ARG, "low-precision matching needs the bf16 dictionary");
+    const bool need_lo = l.mode == MRF_MATCH_SPLIT3 || l.mode == MRF_MATCH_SPLIT2;
+    if (lowp && (!l.lowp_hi || (need_lo && !l.lowp_lo)))
+        return mrf_fail(MRF_E_ARG, "low-precision matching needs the bf16 dictionary%s",
+                        need_lo ? " and its lo part" : "");
+    if (lowp && l.T > kLowpMaxT)
+        return mrf_fail(MRF_E_ARG, "low-precision matching supports T <= %d", kLowpMaxT);
     const int ns = l.vox_slice ? l.n_slices : 1;
     const size_t nat = l.vox_slice ? l.natoms : l.natoms * (size_t)l.n_slices;
     if ((size_t)ns * nat > 0xfffffffeULL)
@@ -837,11 +896,13 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     zero_i32_kernel<<<nb, 256, 0, s>>>(sc.counts, (size_t)ns);
     zero_i32_kernel<<<nb, 256, 0, s>>>(sc.fill, (size_t)ns);
     count_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, ns, sc.counts);
-    tiles_kernel<<<1, 1, 0, s>>>(sc.counts, ns, sc.offsets, sc.tiles, sc.ntiles);
+    tiles_kernel<<<1, 1, 0, s>>>(sc.counts, ns, l.mode == MRF_MATCH_FP32 ? VT : LVT, sc.offsets,
+                                 sc.tiles, sc.ntiles);
     scatter_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, ns, sc.offsets, sc.fill, sc.perm);
 
     if (l.mode != MRF_MATCH_FP32) {
-        if (int rc = launch_lowp_prepare(l.sig, l.V, l.T, sc.shi,
+        const int ld = (int)lowp_ld(l.T);
+        if (int rc = launch_lowp_prepare(l.sig, sc.perm, l.V, l.T, ld, sc.shi,
                                          l.mode == MRF_MATCH_SPLIT3 ? sc.slo : nullptr, s))
             return rc;
         LowpArgs b{};
@@ -857,13 +918,22 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         b.natoms = nat;
         b.chunk = sc.chunk;
         b.V = l.V;
-        b.T = l.T;
-        dim3 grid((unsigned)sc.max_tiles, (unsigned)sc.nchunks);
-        if (l.mode == MRF_MATCH_SPLIT3) match_lowp_kernel<true, 16><<<grid, kBlock, 0, s>>>(b);
-        else match_lowp_kernel<false, 32><<<grid, kBlock, 0, s>>>(b);
-        rescore_kernel<<<dim3((unsigned)((l.V + 3) / 4)), kBlock, 0, s>>>(
-            sc.part_s, sc.part_i, (int)sc.nchunks, l.atoms, l.sig, l.params, l.V, l.T, l.maps,
-            l.idx);
+        b.ld = ld;
+        b.nk = (int)((l.T + LKC - 1) / LKC);
+        b.nch = (int)sc.nchunks;
+        const size_t nwg = (sc.max_tiles * sc.nchunks + 7) / 8 * 8;
+        if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
+        const dim3 grid((unsigned)nwg);
+        if (l.mode == MRF_MATCH_SPLIT3) match_lowp_kernel<3><<<grid, kBlock, 0, s>>>(b);
+        else if (l.mode == MRF_MATCH_SPLIT2) match_lowp_kernel<2><<<grid, kBlock, 0, s>>>(b);
+        else match_lowp_kernel<1><<<grid, kBlock, 0, s>>>(b);
+        const dim3 rg((unsigned)((l.V + 3) / 4));
+        if (l.mode == MRF_MATCH_BF16_TOP16)
+            rescore_kernel<16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks, l.atoms,
+                                                     l.sig, l.params, l.V, l.T, l.maps, l.idx);
+        else
+            rescore_kernel<8><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks, l.atoms,
+                                                    l.sig, l.params, l.V, l.T, l.maps, l.idx);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }

@@ -166,12 +166,18 @@ def test_pd_merge_is_global_first_index_argmax(gpu):
 
 # --------------------------------------------------------------- BF16 ----
 
+LOWP_MODES = (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_SPLIT2,
+              sm.MRF_MATCH_SPLIT3)
+SPLIT_MODES = (sm.MRF_MATCH_SPLIT2, sm.MRF_MATCH_SPLIT3)
+
+
 def match_lowp(torch, atoms, params, sig, natoms, mode, b1map=None):
     """BF16 (top-8 + FP32 rescore) or split-BF16 matching through the C ABI."""
     A, T = atoms.shape
     d_atoms = dev(torch, atoms)
-    d_hi = torch.empty((A, T), dtype=torch.int32, device="cuda")
-    d_lo = torch.empty((A, T), dtype=torch.int32, device="cuda") if mode == sm.MRF_MATCH_SPLIT3 else None
+    ld = sm.lowp_ld(T)
+    d_hi = torch.empty((A, ld), dtype=torch.int32, device="cuda")
+    d_lo = torch.empty((A, ld), dtype=torch.int32, device="cuda") if mode in SPLIT_MODES else None
     sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo)
     V = sig.shape[0]
     d_params, d_sig = dev(torch, params), dev(torch, sig)
@@ -191,11 +197,15 @@ def test_lowp_prepare_rounding(gpu):
     x = (rng.normal(size=(7, 33)) + 1j * rng.normal(size=(7, 33))).astype(np.complex64)
     x[0, :4] = [0, 1e-30, -2.5, 3.0000002]
     d = dev(gpu, x)
-    hi = gpu.empty((7, 33), dtype=gpu.int32, device="cuda")
-    lo = gpu.empty((7, 33), dtype=gpu.int32, device="cuda")
+    ld = sm.lowp_ld(33)
+    assert ld == 64 and sm.lowp_ld(32) == 32 and sm.lowp_ld(1000) == 1024
+    hi = gpu.full((7, ld), -1, dtype=gpu.int32, device="cuda")
+    lo = gpu.full((7, ld), -1, dtype=gpu.int32, device="cuda")
     sm.lowp_prepare(d, 33, 7, hi, lo)
     h = hi.cpu().numpy().view(np.uint32)
     l = lo.cpu().numpy().view(np.uint32)
+    assert not h[:, 33:].any() and not l[:, 33:].any()   # zero pad
+    h, l = h[:, :33], l[:, :33]
 
     def rne(f):
         u = f.view(np.uint32).astype(np.uint64)
@@ -214,7 +224,7 @@ def test_lowp_prepare_rounding(gpu):
     assert np.all(np.abs(rec - re) <= np.abs(re) * 2.0 ** -15 + 1e-38)
 
 
-@pytest.mark.parametrize("mode", [sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_SPLIT3])
+@pytest.mark.parametrize("mode", LOWP_MODES)
 def test_lowp_agrees_with_fp32_and_oracle(gpu, mode):
     """B1-sliced SSFP dictionary, noisy voxels: the rescored argmax agrees
     with the FP32 matcher on >= 99.9% of voxels and equals the FP64 oracle on
@@ -273,6 +283,6 @@ def test_lowp_ragged(gpu, T, V, A):
     atoms /= np.linalg.norm(atoms, axis=1, keepdims=True)
     params = np.stack([np.arange(A), np.zeros(A), np.zeros(A), np.ones(A)], 1).astype(np.float32)
     sig, truth, pd = synth.signals_np(atoms, V, seed=5, snr=8.0)
-    for mode in (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_SPLIT3):
+    for mode in LOWP_MODES:
         maps, idx = match_lowp(gpu, atoms, params, sig, A, mode)
         _check_against_oracle(maps, idx, atoms, params, sig)
