@@ -187,24 +187,31 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                             "frac": res["bf16_top16"]["tflops_algorithmic"] / PEAK_BF16,
                             "traffic": measured_traffic("match_bf16"),
                             "kernel": "match_lowp_kernel<1> + rescore_kernel<16>"}
-    # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path
+    # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path, and
+    # on a sample of the voxels where a low-precision pick differs from the
+    # FP32 pick: which of the two the exact (FP64) argmax chooses.
     if oracle_voxels:
         import oracle
+        atoms = d_atoms.cpu().numpy()
+        b1_host = d_b1.cpu().numpy()
+        slice_b1 = params[::natoms, 3]
+
+        def fp64_match(vox):
+            sig = d_sig[torch.from_numpy(vox).cuda()].cpu().numpy()
+            ks = np.array([oracle.b1_slice(v, slice_b1) for v in b1_host[vox]])
+            ref = np.zeros((len(vox), 5))
+            ridx = np.zeros(len(vox), np.int64)
+            for k in np.unique(ks):
+                sel = ks == k
+                i, pd, _ = oracle.match_slice(atoms[k * natoms:(k + 1) * natoms], sig[sel])
+                ridx[sel] = i + k * natoms
+                ref[sel, :4] = params[ridx[sel]]
+                ref[sel, 4] = pd
+            return ridx, ref
         rng = np.random.default_rng(7)
         sub = np.sort(rng.choice(V, oracle_voxels, replace=False))
-        sig = d_sig[torch.from_numpy(sub).cuda()].cpu().numpy()
-        b1 = d_b1.cpu().numpy()[sub]
-        atoms = d_atoms.cpu().numpy()
-        slice_b1 = params[::natoms, 3]
-        ref = np.zeros((oracle_voxels, 5))
         t0 = time.time()
-        for k in range(A // natoms):
-            sel = np.array([oracle.b1_slice(v, slice_b1) == k for v in b1])
-            if not sel.any():
-                continue
-            ridx, rpd, _ = oracle.match_slice(atoms[k * natoms:(k + 1) * natoms], sig[sel])
-            ref[sel, :4] = params[ridx + k * natoms]
-            ref[sel, 4] = rpd
+        _, ref = fp64_match(sub)
         log(f"FP64 oracle on {oracle_voxels} voxels: {time.time() - t0:.1f} s")
         errs = {}
         for name in ("fp32",) + lowp_names:
@@ -215,6 +222,25 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                           "PD_max_rel": float(np.max(np.abs(m[:, 4] - ref[:, 4]) / ref[:, 4])),
                           "map_exact": float(np.mean(np.all(m[:, :4] == ref[:, :4], axis=1)))}
         res["fp64_oracle_subset"] = {"voxels": oracle_voxels, "errors": errs}
+        arb = {}
+        for name in lowp_names:
+            diff = np.nonzero(out[name][1] != i32)[0]
+            if len(diff) == 0:
+                arb[name] = {"disagree": 0}
+                continue
+            samp = np.sort(rng.choice(diff, min(len(diff), 256), replace=False))
+            t0 = time.time()
+            ridx, _ = fp64_match(samp)
+            lowp_right = float(np.mean(out[name][1][samp] == ridx))
+            fp32_right = float(np.mean(i32[samp] == ridx))
+            arb[name] = {"disagree": len(diff), "sampled": len(samp),
+                         "fp64_picks_lowp": lowp_right, "fp64_picks_fp32": fp32_right,
+                         "agree_fp32_or_fp64": 1.0 - len(diff) / V * (1.0 - lowp_right)}
+            res[name]["agree_fp32_or_fp64"] = arb[name]["agree_fp32_or_fp64"]
+            log(f"FP64 arbitration {name}: {len(diff)} disagreements, lowp right "
+                f"{lowp_right:.3f}, fp32 right {fp32_right:.3f} ({time.time() - t0:.1f} s)")
+        res["fp64_arbitration"] = arb
+        del atoms
     del d_atoms, d_hi, d_lo, d_sig
     torch.cuda.empty_cache()
     return res

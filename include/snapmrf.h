@@ -146,9 +146,9 @@ int mrf_match(float *d_maps, const mrf_complex *h_img,
 
 /* Matching precisions of mrf_match_ex */
 #define MRF_MATCH_FP32 0        /* complex64 dictionary, FP32 MFMA */
-#define MRF_MATCH_BF16_TOP8 1   /* BF16 MFMA + top-8 + FP32 rescore */
+#define MRF_MATCH_BF16_TOP8 1   /* BF16 MFMA + top-8 + rescore of the candidates (FP64 sums) */
 #define MRF_MATCH_SPLIT3 2      /* split-BF16 3 products (hi*hi + lo*hi + hi*lo) + top-8 + rescore */
-#define MRF_MATCH_BF16_TOP16 3  /* BF16 MFMA + top-16 + FP32 rescore */
+#define MRF_MATCH_BF16_TOP16 3  /* BF16 MFMA + top-16 + rescore */
 #define MRF_MATCH_SPLIT2 4      /* dictionary split (a_hi + a_lo) * s_hi + top-8 + rescore */
 
 /* Extended matching, all device-resident, on a caller stream.
