@@ -182,6 +182,52 @@ int mrf_lowp_prepare(void *stream, const mrf_complex *d_atoms, size_t T,
  * a multiple of 32. */
 size_t mrf_lowp_ld(size_t T);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiation form                                                       */
+/* ------------------------------------------------------------------------ */
+/* No reference counterpart: the reference generates forward atoms only
+ * (src/functions.cu:936-1019).  These entry points differentiate the
+ * epg_ssfp program as a continuous function: per atom theta = (T1 ms, T2 ms,
+ * B0 Hz, B1) [n][4] (device), flip angles alpha_n * B1 with the CSV unit
+ * transform of src/functions.cu:1242-1261, unnormalised readouts s_n = F+_0.
+ * h_mrf is the host [4][NREPS] CSV table (FA | phase | TR | TE) of which
+ * nreps rows are used; sequence gradients are with respect to the raw FA and
+ * TR columns (TE held fixed).  2 <= nstates <= 64.  Asynchronous on the
+ * caller stream (NULL = null stream). */
+
+/* d_sig [n][nreps] (optional) and d_jac [n][4][nreps] = ds/dtheta (optional). */
+int mrf_epg_jacobian(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                     const float *d_theta, size_t n, int nstates, mrf_complex *d_sig,
+                     mrf_complex *d_jac);
+
+/* Reverse mode for L = sum_atoms Re sum_n conj(cot_n) s_n with d_cot
+ * [n][nreps]: d_grad_theta [n][4] = dL/dtheta (optional) and d_grad_seq
+ * [2][nreps] += (dL/dFA_raw, dL/dTR_raw) summed over atoms (optional,
+ * accumulated: zero it first).  States are checkpointed every 50 TRs and
+ * recomputed per segment. */
+int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                    const float *d_theta, size_t n, int nstates, const mrf_complex *d_cot,
+                    float *d_grad_theta, float *d_grad_seq);
+
+/* Per-voxel Gauss-Newton (Levenberg-Marquardt damped, variable projection of
+ * the complex proton density) refinement of d_theta [V][4] (in: start, e.g.
+ * the matched grid point; out: refined) against d_sig [V][nreps], iters
+ * iterations; the cost |x|^2 - |s^H x|^2/|s|^2 never increases.  d_pd [V]
+ * (optional) = |s^H x|/|s|; d_cost [V][iters + 1] (optional) cost history.
+ * The 4x4 normal equations are solved in FP64. */
+int mrf_gn_refine(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                  const mrf_complex *d_sig, size_t V, int nstates, int iters, float *d_theta,
+                  float *d_pd, float *d_cost);
+
+/* Cramer-Rao-bound sequence-design loss over n tissues d_tissues [n][4]:
+ * d_loss[i] = sum_d wts[d] [F_i^-1]_dd / scale_d^2, F_i = Re(J_i^H J_i) (unit
+ * noise and proton density), scale = (T1, T2, 1, 1), inverted in FP64 (NaN
+ * if singular); d_grad_seq [2][nreps] += d(sum_i loss_i)/d(FA_raw, TR_raw)
+ * (accumulated: zero it first).  wts is host [4]. */
+int mrf_crb_loss_grad(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                      const float *d_tissues, size_t n, int nstates, const float *wts,
+                      double *d_loss, float *d_grad_seq);
+
 /* Device count visible to the library (0 when no GPU). */
 int mrf_device_count(void);
 

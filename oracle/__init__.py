@@ -42,6 +42,12 @@ def lib():
         L.orc_match_slice.argtypes = [P, S, S, P, S, P, P, P]
         L.orc_b1_slice.argtypes = [ctypes.c_float, P, ctypes.c_int]
         L.orc_b1_slice.restype = ctypes.c_int
+        L.orc_dm_jacobian.argtypes = [P, S, P, ctypes.c_int, P, P]
+        L.orc_dm_adjoint.argtypes = [P, S, P, ctypes.c_int, P, P, P, P]
+        L.orc_dm_crb.argtypes = [P, S, P, S, ctypes.c_int, P, P]
+        L.orc_dm_crb.restype = ctypes.c_int
+        L.orc_inv4.argtypes = [P, P]
+        L.orc_inv4.restype = ctypes.c_int
         L.orc_wall.restype = ctypes.c_double
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
@@ -109,3 +115,49 @@ def wall() -> float:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+# ---- differentiable EPG-SSFP model (diff.c) -------------------------------
+
+def dm_jacobian(raw: np.ndarray, theta, w: int, jac: bool = True):
+    """Continuous FP64 EPG-SSFP signal s [T] (complex128) and, if jac, the
+    Jacobian dS/d(T1, T2, B0, B1) [4, T] (complex128)."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    T = raw.shape[1]
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    s = np.empty(T, np.complex128)
+    J = np.empty((4, T), np.complex128) if jac else None
+    lib().orc_dm_jacobian(_p(raw), T, _p(th), w, _p(s), _p(J) if jac else None)
+    return (s, J) if jac else s
+
+
+def dm_adjoint(raw: np.ndarray, theta, w: int, v: np.ndarray):
+    """Gradients of Re sum conj(v) s: (g_theta [4], g_fa [T], g_tr [T])."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    T = raw.shape[1]
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    g = np.empty(4)
+    gfa = np.empty(T)
+    gtr = np.empty(T)
+    lib().orc_dm_adjoint(_p(raw), T, _p(th), w, _p(v), _p(g), _p(gfa), _p(gtr))
+    return g, gfa, gtr
+
+
+def dm_crb(raw: np.ndarray, tissues: np.ndarray, w: int, wts) -> float:
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    tis = np.ascontiguousarray(tissues, dtype=np.float64)
+    wt = np.ascontiguousarray(wts, dtype=np.float64)
+    out = np.zeros(1)
+    rc = lib().orc_dm_crb(_p(raw), raw.shape[1], _p(tis), tis.shape[0], w, _p(wt), _p(out))
+    if rc:
+        raise ValueError("singular Fisher matrix")
+    return float(out[0])
+
+
+def inv4(A: np.ndarray) -> np.ndarray:
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    X = np.empty((4, 4))
+    if lib().orc_inv4(_p(A), _p(X)):
+        raise ValueError("singular")
+    return X

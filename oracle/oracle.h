@@ -75,6 +75,29 @@ void orc_match_slice(const float *atoms, size_t T, size_t na,
                      const float *sig, size_t V,
                      long long *best_idx, double *best_pd, double *second_pd);
 
+/* ---- differentiable EPG-SSFP model (diff.c) ------------------------------
+ * Continuous FP64 form of the epg_ssfp program (functions.cu:936-1019): raw
+ * CSV columns [4][T] (float) with the unit transform applied in double, one
+ * atom theta = (T1 ms, T2 ms, B0 Hz, B1), flip angles alpha_n * B1.  Signals
+ * are the unnormalised F+_0 readouts, complex interleaved doubles. */
+
+/* s [T] and (J != NULL) dS/dtheta [4][T] by forward-mode propagation. */
+void orc_dm_jacobian(const float *raw, size_t T, const double *theta, int w,
+                     double *s, double *J);
+
+/* L = Re sum_n conj(v_n) s_n: dL/dtheta [4], dL/dFA_raw [T], dL/dTR_raw [T]
+ * (TE held fixed) by reverse mode over stored TR-start states. */
+void orc_dm_adjoint(const float *raw, size_t T, const double *theta, int w,
+                    const double *v, double *g_theta, double *g_fa, double *g_tr);
+
+/* Cramer-Rao-bound loss sum_i sum_d wts[d] [F_i^-1]_dd / scale_d^2 with
+ * F_i = Re(J_i^H J_i), scale = (T1, T2, 1, 1); returns -1 for a singular F. */
+int orc_dm_crb(const float *raw, size_t T, const double *tissues, size_t n, int w,
+               const double *wts, double *loss);
+
+/* 4x4 inverse, Gauss-Jordan with partial pivoting (-1 if singular). */
+int orc_inv4(const double *A, double *X);
+
 /* B1-map merge (functions.cu:1351-1384): slice k = argmin_k |b1map - B1_k|,
  * strict <, init 10 (float32 arithmetic as in the reference). */
 int orc_b1_slice(float b1map_v, const float *slice_b1, int l_b1);

@@ -30,7 +30,8 @@ EXPORTS = (
     "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
     "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
     "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
-    "mrf_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_device_count",
+    "mrf_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
+    "mrf_gn_refine", "mrf_crb_loss_grad", "mrf_device_count",
 )
 
 
@@ -84,6 +85,10 @@ def lib() -> ctypes.CDLL:
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
         "mrf_lowp_prepare": ([P, P, S, S, P, P], I),
         "mrf_lowp_ld": ([S], S),
+        "mrf_epg_jacobian": ([P, P, S, S, P, S, I, P, P], I),
+        "mrf_epg_adjoint": ([P, P, S, S, P, S, I, P, P, P], I),
+        "mrf_gn_refine": ([P, P, S, S, P, S, I, I, P, P, P], I),
+        "mrf_crb_loss_grad": ([P, P, S, S, P, S, I, P, P, P], I),
         "mrf_device_count": ([], I),
     }
     for name, (args, res) in sig.items():
@@ -284,3 +289,48 @@ def lowp_prepare(d_atoms, T: int, NATOMS: int, d_hi, d_lo=None, stream=None) -> 
     """d_hi/d_lo: device buffers of NATOMS * lowp_ld(T) 32-bit words."""
     check(lib().mrf_lowp_prepare(_stream_ptr(stream), _ptr(d_atoms), T, NATOMS, _ptr(d_hi),
                                  _ptr(d_lo)))
+
+
+# ---- differentiation form ---------------------------------------------------
+
+def _seq_table(h_mrf):
+    """Host [4][T] float32 CSV table -> (array, T)."""
+    import numpy as np
+    h = np.ascontiguousarray(h_mrf, dtype=np.float32)
+    if h.ndim != 2 or h.shape[0] != 4:
+        raise ValueError("h_mrf must be [4][T]")
+    return h, h.shape[1]
+
+
+def epg_jacobian(h_mrf, d_theta, n: int, nstates: int, d_sig=None, d_jac=None,
+                 stream=None) -> None:
+    """Unnormalised EPG-SSFP signals d_sig [n][T] and ds/d(T1,T2,B0,B1)
+    d_jac [n][4][T] (complex64 device buffers, either optional)."""
+    h, T = _seq_table(h_mrf)
+    check(lib().mrf_epg_jacobian(_stream_ptr(stream), h.ctypes.data, T, T, _ptr(d_theta), n,
+                                 nstates, _ptr(d_sig), _ptr(d_jac)))
+
+
+def epg_adjoint(h_mrf, d_theta, n: int, nstates: int, d_cot, d_grad_theta=None,
+                d_grad_seq=None, stream=None) -> None:
+    """Reverse mode of L = sum Re(conj(cot) s): d_grad_theta [n][4] and
+    d_grad_seq [2][T] += (dL/dFA_raw, dL/dTR_raw)."""
+    h, T = _seq_table(h_mrf)
+    check(lib().mrf_epg_adjoint(_stream_ptr(stream), h.ctypes.data, T, T, _ptr(d_theta), n,
+                                nstates, _ptr(d_cot), _ptr(d_grad_theta), _ptr(d_grad_seq)))
+
+
+def gn_refine(h_mrf, d_sig, V: int, nstates: int, iters: int, d_theta, d_pd=None,
+              d_cost=None, stream=None) -> None:
+    h, T = _seq_table(h_mrf)
+    check(lib().mrf_gn_refine(_stream_ptr(stream), h.ctypes.data, T, T, _ptr(d_sig), V, nstates,
+                              iters, _ptr(d_theta), _ptr(d_pd), _ptr(d_cost)))
+
+
+def crb_loss_grad(h_mrf, d_tissues, n: int, nstates: int, wts, d_loss, d_grad_seq=None,
+                  stream=None) -> None:
+    import numpy as np
+    h, T = _seq_table(h_mrf)
+    w = np.ascontiguousarray(wts, dtype=np.float32)
+    check(lib().mrf_crb_loss_grad(_stream_ptr(stream), h.ctypes.data, T, T, _ptr(d_tissues), n,
+                                  nstates, w.ctypes.data, _ptr(d_loss), _ptr(d_grad_seq)))

@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsnapmrf.so")
 CLI = os.path.join(HERE, "bin", "mrf")
 
-HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "capi.hip"]
+HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "grad.hip", "capi.hip"]
 HOST_SOURCES = ["host.cpp"]
 ARCH = "gfx950"
 

@@ -320,3 +320,159 @@ extern "C" size_t mrf_lowp_ld(size_t T)
 {
     return lowp_ld(T);
 }
+
+// ---------------------------------------------------------------------------
+// Differentiation form
+
+namespace mrf {
+namespace {
+
+// Per-TR table of the continuous model: CSV unit transform
+// (src/functions.cu:1242-1261) in double.
+void build_dtr(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<DTr> &out)
+{
+    out.resize(nreps);
+    for (size_t i = 0; i < nreps; i++) {
+        const double fa = (double)h_mrf[i] * (60.0 * M_PI / 180.0);
+        const double ph = (double)h_mrf[NREPS + i] * (M_PI / 180.0);
+        const double tr = (double)h_mrf[2 * NREPS + i] + 16.0;
+        const double te = (double)h_mrf[3 * NREPS + i] + 3.5;
+        DTr d;
+        d.al = (float)fa;
+        d.cph = (float)cos(ph);
+        d.sph = (float)sin(ph);
+        d.c2p = (float)cos(2.0 * ph);
+        d.s2p = (float)sin(2.0 * ph);
+        d.te = (float)te;
+        d.t2 = (float)(tr - te);
+        d.pad = 0.f;
+        out[i] = d;
+    }
+}
+
+constexpr float kFaUnit = (float)(60.0 * M_PI / 180.0);
+constexpr size_t kDiffScratchBudget = (size_t)1 << 30;   // bytes per launch batch
+
+int diff_check(const float *h_mrf, size_t nreps, size_t NREPS, int nstates)
+{
+    if (!h_mrf) return mrf_fail(MRF_E_ARG, "null sequence table");
+    if (nreps == 0 || nreps > NREPS || nreps > 0x7fffffff)
+        return mrf_fail(MRF_E_ARG, "need 0 < nreps <= NREPS");
+    if (nstates < 2 || nstates > 64)
+        return mrf_fail(MRF_E_ARG, "differentiation needs 2 <= nstates <= 64 (one lane per state)");
+    return check_device();
+}
+
+// Uploads the per-TR table; *d must be freed by the caller on s.
+int upload_dtr(const float *h_mrf, size_t nreps, size_t NREPS, hipStream_t s, DTr **d)
+{
+    std::vector<DTr> tab;
+    build_dtr(h_mrf, nreps, NREPS, tab);
+    MRF_HIP(hipMallocAsync((void **)d, nreps * sizeof(DTr), s));
+    MRF_HIP(hipMemcpyAsync(*d, tab.data(), nreps * sizeof(DTr), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+}  // namespace
+}  // namespace mrf
+
+extern "C" int mrf_epg_jacobian(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                                const float *d_theta, size_t n, int nstates, mrf_complex *d_sig,
+                                mrf_complex *d_jac)
+{
+    mrf_clear_error();
+    if (int rc = diff_check(h_mrf, nreps, NREPS, nstates)) return rc;
+    if (n && !d_theta) return mrf_fail(MRF_E_ARG, "null theta");
+    if (n == 0 || (!d_sig && !d_jac)) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    DTr *d_seq = nullptr;
+    if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    int rc = launch_jacobian(d_seq, (int)nreps, nstates, d_theta, n,
+                             reinterpret_cast<float2 *>(d_sig), reinterpret_cast<float2 *>(d_jac), s);
+    (void)hipFreeAsync(d_seq, s);
+    return rc;
+}
+
+extern "C" int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                               const float *d_theta, size_t n, int nstates,
+                               const mrf_complex *d_cot, float *d_grad_theta, float *d_grad_seq)
+{
+    mrf_clear_error();
+    if (int rc = diff_check(h_mrf, nreps, NREPS, nstates)) return rc;
+    if (n && (!d_theta || !d_cot)) return mrf_fail(MRF_E_ARG, "null theta or cotangent");
+    if (n == 0 || (!d_grad_theta && !d_grad_seq)) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int T = (int)nreps;
+    const size_t per = diff_scratch_floats(0, T, nstates) * sizeof(float);
+    size_t batch = kDiffScratchBudget / per;
+    if (batch < 1) batch = 1;
+    if (batch > n) batch = n;
+    DTr *d_seq = nullptr;
+    if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    float *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, batch * per, s) != hipSuccess) {
+        (void)hipFreeAsync(d_seq, s);
+        return mrf_fail(MRF_E_NOMEM, "adjoint scratch (%zu bytes)", batch * per);
+    }
+    int rc = 0;
+    for (size_t a0 = 0; a0 < n && !rc; a0 += batch) {
+        const size_t m = (n - a0 < batch) ? n - a0 : batch;
+        rc = launch_adjoint(d_seq, T, nstates, d_theta + 4 * a0, m,
+                            reinterpret_cast<const float2 *>(d_cot) + a0 * nreps,
+                            d_grad_theta ? d_grad_theta + 4 * a0 : nullptr, d_grad_seq, kFaUnit,
+                            scratch, s);
+    }
+    (void)hipFreeAsync(scratch, s);
+    (void)hipFreeAsync(d_seq, s);
+    return rc;
+}
+
+extern "C" int mrf_gn_refine(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                             const mrf_complex *d_sig, size_t V, int nstates, int iters,
+                             float *d_theta, float *d_pd, float *d_cost)
+{
+    mrf_clear_error();
+    if (int rc = diff_check(h_mrf, nreps, NREPS, nstates)) return rc;
+    if (V && (!d_sig || !d_theta)) return mrf_fail(MRF_E_ARG, "null signal or theta");
+    if (iters < 0 || iters > 1000) return mrf_fail(MRF_E_ARG, "iters must be in [0, 1000]");
+    if (V == 0) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    DTr *d_seq = nullptr;
+    if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    int rc = launch_gn(d_seq, (int)nreps, nstates, reinterpret_cast<const float2 *>(d_sig), V,
+                       d_theta, iters, d_pd, d_cost, s);
+    (void)hipFreeAsync(d_seq, s);
+    return rc;
+}
+
+extern "C" int mrf_crb_loss_grad(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
+                                 const float *d_tissues, size_t n, int nstates, const float *wts,
+                                 double *d_loss, float *d_grad_seq)
+{
+    mrf_clear_error();
+    if (int rc = diff_check(h_mrf, nreps, NREPS, nstates)) return rc;
+    if (n && (!d_tissues || !d_loss || !wts)) return mrf_fail(MRF_E_ARG, "null argument");
+    if (n == 0) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int T = (int)nreps;
+    const size_t per = diff_scratch_floats(4, T, nstates) * sizeof(float);
+    size_t batch = kDiffScratchBudget / per;
+    if (batch < 1) batch = 1;
+    if (batch > n) batch = n;
+    DTr *d_seq = nullptr;
+    if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    float *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, batch * per, s) != hipSuccess) {
+        (void)hipFreeAsync(d_seq, s);
+        return mrf_fail(MRF_E_NOMEM, "crb scratch (%zu bytes)", batch * per);
+    }
+    int rc = 0;
+    for (size_t a0 = 0; a0 < n && !rc; a0 += batch) {
+        const size_t m = (n - a0 < batch) ? n - a0 : batch;
+        rc = launch_crb(d_seq, T, nstates, d_tissues + 4 * a0, m, wts, d_loss + a0, d_grad_seq,
+                        kFaUnit, scratch, s);
+    }
+    (void)hipFreeAsync(scratch, s);
+    (void)hipFreeAsync(d_seq, s);
+    return rc;
+}

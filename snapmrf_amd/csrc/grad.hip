@@ -1,0 +1,922 @@
+// grad.hip -- differentiation form of the EPG-SSFP program: Jacobians with
+// respect to (T1, T2, B0, B1), reverse-mode adjoints with respect to those
+// and to the per-timepoint flip angles and repetition times (checkpointed
+// recomputation every kCkpt TRs), per-voxel Gauss-Newton refinement, and the
+// gradient of a Cramer-Rao-bound sequence-design loss.
+//
+// Model: the reference's epg_ssfp program (src/functions.cu:936-1019 with
+// init_rf_pulse :456-481, apply_rf_pulse :483-528, decay_signal :530-583,
+// shift_phase :585-607, dephase_gradients :619-668) as a continuous function
+// of its inputs; the flip angles of an atom are alpha_n * B1, the CSV unit
+// transform (functions.cu:1242-1261) is folded into the per-TR table.
+//
+// Layout: one wave per atom, lane k holds EPG state k (F+_k, F-_k, Z_k) and
+// its ND tangents (w <= 64), so an atom's whole state sits in 6 (1 + ND)
+// registers per lane; the dephasing shift is a cross-lane move.  Per TR n:
+//   RF(alpha_n B1, phi_n) -> relax(TE_n) -> phase(TE_n) -> s_n = F+_0 (lane 0)
+//   -> phase(TR_n - TE_n) -> dephase -> relax(TR_n - TE_n)
+// Arithmetic is FP32; FP64 is used only for the 4x4 Gauss-Newton solve and
+// the Fisher-matrix inverse.
+#include "kernels.h"
+#include "snapmrf.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace mrf {
+
+namespace {
+
+constexpr int kCkpt = 50;                 // TRs between checkpoints
+constexpr int kWaves = 4;                 // atoms per work-group
+constexpr float kCB0 = (float)(2.0 * M_PI / 1000.0);
+constexpr float kTI = 40.0f;              // inversion delay, functions.cu:1242
+
+// complex float helpers
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ float2 c_conj(float2 a) { return {a.x, -a.y}; }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ float2 c_i(float2 a) { return {-a.y, a.x}; }      // i * a
+__device__ __forceinline__ float re_dot(float2 l, float2 v) { return l.x * v.x + l.y * v.y; }  // Re(conj(l) v)
+
+__device__ __forceinline__ float2 shfl_up1(float2 v)
+{
+    return {__shfl_up(v.x, 1), __shfl_up(v.y, 1)};
+}
+__device__ __forceinline__ float2 shfl_down1(float2 v)
+{
+    return {__shfl_down(v.x, 1), __shfl_down(v.y, 1)};
+}
+__device__ __forceinline__ float2 bcast0(float2 v) { return {__shfl(v.x, 0), __shfl(v.y, 0)}; }
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// RF matrix entries for angle a (or its a-derivatives)
+struct C4 {
+    float ca2, sa2, sa, ca;
+};
+
+// y' = M y with M(a, phi) of apply_rf_pulse (functions.cu:483-528)
+__device__ __forceinline__ void rf3(const C4 &c, float2 zz, float2 z2, float2 fp, float2 fm,
+                                    float2 z, float2 &op, float2 &om, float2 &oz)
+{
+    const float2 zc = c_conj(zz);
+    op = c_add(c_add(c_scale(fp, c.ca2), c_scale(c_mul(z2, fm), c.sa2)),
+               c_scale(c_i(c_mul(zz, z)), -c.sa));
+    om = c_add(c_add(c_scale(c_mul(c_conj(z2), fp), c.sa2), c_scale(fm, c.ca2)),
+               c_scale(c_i(c_mul(zc, z)), c.sa));
+    oz = c_add(c_add(c_scale(c_i(c_mul(zc, fp)), -0.5f * c.sa), c_scale(c_i(c_mul(zz, fm)), 0.5f * c.sa)),
+               c_scale(z, c.ca));
+}
+
+// l = M^H l'
+__device__ __forceinline__ void rf3_adj(const C4 &c, float2 zz, float2 z2, float2 lp, float2 lm,
+                                        float2 lz, float2 &op, float2 &om, float2 &oz)
+{
+    const float2 zc = c_conj(zz);
+    op = c_add(c_add(c_scale(lp, c.ca2), c_scale(c_mul(z2, lm), c.sa2)),
+               c_scale(c_i(c_mul(zz, lz)), 0.5f * c.sa));
+    om = c_add(c_add(c_scale(c_mul(c_conj(z2), lp), c.sa2), c_scale(lm, c.ca2)),
+               c_scale(c_i(c_mul(zc, lz)), -0.5f * c.sa));
+    oz = c_add(c_add(c_scale(c_i(c_mul(zc, lp)), c.sa), c_scale(c_i(c_mul(zz, lm)), -c.sa)),
+               c_scale(lz, c.ca));
+}
+
+__device__ __forceinline__ C4 rf_c(float sa, float ca) { return {0.5f * (1.f + ca), 0.5f * (1.f - ca), sa, ca}; }
+__device__ __forceinline__ C4 rf_d1(float sa, float ca) { return {-0.5f * sa, 0.5f * sa, ca, -sa}; }
+__device__ __forceinline__ C4 rf_d2(float sa, float ca) { return {-0.5f * ca, 0.5f * ca, -sa, -ca}; }
+
+struct Atom {
+    float T1, T2, B0, B1;
+    float iT1, iT2;
+};
+
+// Augmented state: index 0 is the signal state, 1 + d the tangent along
+// theta_d, d = 0 (T1), 1 (T2), 2 (B0), 3 (B1).
+template <int ND>
+struct Aug {
+    float2 fp[1 + ND], fm[1 + ND], z[1 + ND];
+};
+
+// Per-TR coefficients, identical on all lanes of the wave.
+struct TrK {
+    float sa, ca;            // sin/cos of alpha_n B1
+    float2 zz, z2;
+    float al;
+    float e1a, e2a, e1b, e2b;     // relaxation over TE and TR - TE
+    float2 pa, pb;                // e^{i beta TE}, e^{i beta (TR - TE)}
+    float te, t2;
+};
+
+__device__ __forceinline__ TrK tr_coef(const DTr &q, const Atom &p)
+{
+    TrK c;
+    c.al = q.al;
+    sincosf(q.al * p.B1, &c.sa, &c.ca);
+    c.zz = {q.cph, q.sph};
+    c.z2 = {q.c2p, q.s2p};
+    c.te = q.te;
+    c.t2 = q.t2;
+    c.e1a = expf(-q.te * p.iT1);
+    c.e2a = expf(-q.te * p.iT2);
+    c.e1b = expf(-q.t2 * p.iT1);
+    c.e2b = expf(-q.t2 * p.iT2);
+    const float beta = kCB0 * p.B0;
+    float s, co;
+    sincosf(beta * q.te, &s, &co);
+    c.pa = {co, s};
+    sincosf(beta * q.t2, &s, &co);
+    c.pb = {co, s};
+    return c;
+}
+
+// --- forward operators on the augmented state (lane k) -------------------
+
+template <int ND>
+__device__ __forceinline__ void f_rf(Aug<ND> &y, const TrK &c)
+{
+    const C4 m = rf_c(c.sa, c.ca);
+    float2 sp, sm, sz;
+    if constexpr (ND > 0) {
+        rf3(rf_d1(c.sa, c.ca), c.zz, c.z2, y.fp[0], y.fm[0], y.z[0], sp, sm, sz);
+    }
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        float2 a, b, d;
+        rf3(m, c.zz, c.z2, y.fp[j], y.fm[j], y.z[j], a, b, d);
+        y.fp[j] = a;
+        y.fm[j] = b;
+        y.z[j] = d;
+    }
+    if constexpr (ND > 0) {   // d/dB1: alpha M_a y
+        y.fp[4] = c_add(y.fp[4], c_scale(sp, c.al));
+        y.fm[4] = c_add(y.fm[4], c_scale(sm, c.al));
+        y.z[4] = c_add(y.z[4], c_scale(sz, c.al));
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ void f_relax(Aug<ND> &y, float e1, float e2, float t, const Atom &p,
+                                        bool k0)
+{
+    if constexpr (ND > 0) {
+        const float e1d = e1 * t * p.iT1 * p.iT1, e2d = e2 * t * p.iT2 * p.iT2;
+#pragma unroll
+        for (int j = 1; j <= ND; j++) {
+            y.fp[j] = c_scale(y.fp[j], e2);
+            y.fm[j] = c_scale(y.fm[j], e2);
+            y.z[j] = c_scale(y.z[j], e1);
+        }
+        y.fp[2] = c_add(y.fp[2], c_scale(y.fp[0], e2d));
+        y.fm[2] = c_add(y.fm[2], c_scale(y.fm[0], e2d));
+        float2 zr = y.z[0];
+        if (k0) zr.x -= 1.f;
+        y.z[1] = c_add(y.z[1], c_scale(zr, e1d));
+    }
+    y.fp[0] = c_scale(y.fp[0], e2);
+    y.fm[0] = c_scale(y.fm[0], e2);
+    y.z[0] = c_scale(y.z[0], e1);
+    if (k0) y.z[0].x += 1.f - e1;
+}
+
+template <int ND>
+__device__ __forceinline__ void f_phase(Aug<ND> &y, float2 P, float t)
+{
+    const float2 Q = c_conj(P);
+    if constexpr (ND > 0) {
+#pragma unroll
+        for (int j = 1; j <= ND; j++) {
+            y.fp[j] = c_mul(P, y.fp[j]);
+            y.fm[j] = c_mul(Q, y.fm[j]);
+        }
+        const float ct = kCB0 * t;
+        y.fp[3] = c_add(y.fp[3], c_scale(c_i(c_mul(P, y.fp[0])), ct));
+        y.fm[3] = c_add(y.fm[3], c_scale(c_i(c_mul(Q, y.fm[0])), -ct));
+    }
+    y.fp[0] = c_mul(P, y.fp[0]);
+    y.fm[0] = c_mul(Q, y.fm[0]);
+}
+
+// dephase_gradients (functions.cu:619-668): F+_k <- F+_{k-1}, F-_k <- F-_{k+1},
+// F+_0 <- conj(F-_0), F-_{w-1} <- 0; lanes >= w stay zero.
+template <int ND>
+__device__ __forceinline__ void f_dephase(Aug<ND> &y, int k, int w)
+{
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        float2 m = shfl_down1(y.fm[j]);
+        float2 pl = shfl_up1(y.fp[j]);
+        if (k >= w - 1) m = {0.f, 0.f};
+        if (k == 0) pl = c_conj(m);
+        if (k >= w) pl = {0.f, 0.f};
+        y.fp[j] = pl;
+        y.fm[j] = m;
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ void f_tr(Aug<ND> &y, const TrK &c, const Atom &p, int k, int w,
+                                     Aug<ND> *at_readout)
+{
+    f_rf(y, c);
+    f_relax(y, c.e1a, c.e2a, c.te, p, k == 0);
+    f_phase(y, c.pa, c.te);
+    if (at_readout) *at_readout = y;
+    f_phase(y, c.pb, c.t2);
+    f_dephase(y, k, w);
+    f_relax(y, c.e1b, c.e2b, c.t2, p, k == 0);
+}
+
+// init_rf_pulse(180 deg, 0 deg, TI) (functions.cu:456-481) on lane 0, with
+// the tangents of its T1/T2 dependence.
+template <int ND>
+__device__ __forceinline__ void f_init(Aug<ND> &y, const Atom &p, int k)
+{
+#pragma unroll
+    for (int j = 0; j <= ND; j++) y.fp[j] = y.fm[j] = y.z[j] = {0.f, 0.f};
+    if (k != 0) return;
+    const float a0 = (float)(180.0 * M_PI / 180.0f);
+    const float sa0 = sinf(a0), ca0 = cosf(a0);
+    const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
+    const float2 tmp = {0.f, e2 * sa0};
+    y.fp[0] = c_conj(tmp);
+    y.fm[0] = tmp;
+    y.z[0] = {e1 * ca0 + 1.f - e1, 0.f};
+    if constexpr (ND > 0) {
+        const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
+        y.z[1] = {e1d * (ca0 - 1.f), 0.f};
+        const float2 t2 = {0.f, e2d * sa0};
+        y.fp[2] = c_conj(t2);
+        y.fm[2] = t2;
+    }
+}
+
+// --- reverse operators ----------------------------------------------------
+// l holds the adjoint of the operator's output on entry and of its input on
+// exit; x is the operator's input.  ga / gt collect the lane's share of
+// dL/d(alpha_n) and dL/dt (t the interval of the operator); gth (ND == 0)
+// the share of dL/dtheta.
+
+template <int ND>
+__device__ __forceinline__ void r_relax(Aug<ND> &l, const Aug<ND> &x, float e1, float e2, float t,
+                                        const Atom &p, bool k0, float *gt, float gth[4])
+{
+    const float i1 = p.iT1, i2 = p.iT2;
+    const float e1d = e1 * t * i1 * i1, e2d = e2 * t * i2 * i2;
+    float2 zr = x.z[0];
+    if (k0) zr.x -= 1.f;
+    if (gt) {
+        const float e1t = -e1 * i1, e2t = -e2 * i2;
+        float g = re_dot(l.fp[0], x.fp[0]) * e2t + re_dot(l.fm[0], x.fm[0]) * e2t +
+                  re_dot(l.z[0], zr) * e1t;
+        if constexpr (ND > 0) {
+#pragma unroll
+            for (int j = 1; j <= ND; j++)
+                g += (re_dot(l.fp[j], x.fp[j]) + re_dot(l.fm[j], x.fm[j])) * e2t +
+                     re_dot(l.z[j], x.z[j]) * e1t;
+            const float e1dt = e1 * i1 * i1 * (1.f - t * i1), e2dt = e2 * i2 * i2 * (1.f - t * i2);
+            g += (re_dot(l.fp[2], x.fp[0]) + re_dot(l.fm[2], x.fm[0])) * e2dt +
+                 re_dot(l.z[1], zr) * e1dt;
+        }
+        *gt += g;
+    }
+    if constexpr (ND == 0) {
+        gth[0] += re_dot(l.z[0], zr) * e1d;
+        gth[1] += (re_dot(l.fp[0], x.fp[0]) + re_dot(l.fm[0], x.fm[0])) * e2d;
+    }
+    float2 lp0 = c_scale(l.fp[0], e2), lm0 = c_scale(l.fm[0], e2), lz0 = c_scale(l.z[0], e1);
+    if constexpr (ND > 0) {
+        lp0 = c_add(lp0, c_scale(l.fp[2], e2d));
+        lm0 = c_add(lm0, c_scale(l.fm[2], e2d));
+        lz0 = c_add(lz0, c_scale(l.z[1], e1d));
+#pragma unroll
+        for (int j = 1; j <= ND; j++) {
+            l.fp[j] = c_scale(l.fp[j], e2);
+            l.fm[j] = c_scale(l.fm[j], e2);
+            l.z[j] = c_scale(l.z[j], e1);
+        }
+    }
+    l.fp[0] = lp0;
+    l.fm[0] = lm0;
+    l.z[0] = lz0;
+}
+
+template <int ND>
+__device__ __forceinline__ void r_dephase(Aug<ND> &l, int k, int w)
+{
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        const float2 lp0 = bcast0(l.fp[j]);
+        float2 p = shfl_down1(l.fp[j]);
+        float2 m = shfl_up1(l.fm[j]);
+        if (k >= w - 1) p = {0.f, 0.f};
+        if (k == 0 || k >= w) m = {0.f, 0.f};
+        if (k == 1) m = c_add(m, c_conj(lp0));
+        l.fp[j] = p;
+        l.fm[j] = m;
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ void r_phase(Aug<ND> &l, const Aug<ND> &x, float2 P, float t,
+                                        float beta, float *gt, float gth[4])
+{
+    const float2 Q = c_conj(P);
+    const float ct = kCB0 * t;
+    const float2 Px0 = c_mul(P, x.fp[0]), Qx0 = c_mul(Q, x.fm[0]);
+    if (gt) {
+        // d/dt of P = i beta P, of Q = -i beta Q
+        float g = re_dot(l.fp[0], c_scale(c_i(Px0), beta)) + re_dot(l.fm[0], c_scale(c_i(Qx0), -beta));
+        if constexpr (ND > 0) {
+#pragma unroll
+            for (int j = 1; j <= ND; j++)
+                g += re_dot(l.fp[j], c_scale(c_i(c_mul(P, x.fp[j])), beta)) +
+                     re_dot(l.fm[j], c_scale(c_i(c_mul(Q, x.fm[j])), -beta));
+            // d/dt of the B0 sources i c t P and -i c t Q
+            const float2 a = c_scale(c_i(Px0), kCB0);          // i c P x
+            const float2 bp = c_sub(a, c_scale(Px0, kCB0 * beta * t));   // i c P x (1 + i beta t)
+            const float2 am = c_scale(c_i(Qx0), -kCB0);         // -i c Q x
+            const float2 bm = c_sub(am, c_scale(Qx0, kCB0 * beta * t));  // -i c Q x (1 - i beta t)
+            g += re_dot(l.fp[3], bp) + re_dot(l.fm[3], bm);
+        }
+        *gt += g;
+    }
+    if constexpr (ND == 0)
+        gth[2] += re_dot(l.fp[0], c_scale(c_i(Px0), ct)) + re_dot(l.fm[0], c_scale(c_i(Qx0), -ct));
+    float2 lp0 = c_mul(Q, l.fp[0]), lm0 = c_mul(P, l.fm[0]);
+    if constexpr (ND > 0) {
+        // conj(i c t P) = -i c t Q ; conj(-i c t Q) = i c t P
+        lp0 = c_add(lp0, c_scale(c_i(c_mul(Q, l.fp[3])), -ct));
+        lm0 = c_add(lm0, c_scale(c_i(c_mul(P, l.fm[3])), ct));
+#pragma unroll
+        for (int j = 1; j <= ND; j++) {
+            l.fp[j] = c_mul(Q, l.fp[j]);
+            l.fm[j] = c_mul(P, l.fm[j]);
+        }
+    }
+    l.fp[0] = lp0;
+    l.fm[0] = lm0;
+}
+
+template <int ND>
+__device__ __forceinline__ void r_rf(Aug<ND> &l, const Aug<ND> &x, const TrK &c, float B1,
+                                     float *ga, float gth[4])
+{
+    const C4 m = rf_c(c.sa, c.ca), m1 = rf_d1(c.sa, c.ca);
+    float g_a = 0.f, g_al = 0.f;
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        float2 a, b, d;
+        rf3(m1, c.zz, c.z2, x.fp[j], x.fm[j], x.z[j], a, b, d);
+        g_a += re_dot(l.fp[j], a) + re_dot(l.fm[j], b) + re_dot(l.z[j], d);
+        if (ND > 0 && j == 0) {
+            // explicit alpha of the B1 source alpha M_a x0
+            g_al += re_dot(l.fp[4], a) + re_dot(l.fm[4], b) + re_dot(l.z[4], d);
+        }
+    }
+    float2 sp, sm, sz;
+    if constexpr (ND > 0) {
+        float2 a, b, d;
+        rf3(rf_d2(c.sa, c.ca), c.zz, c.z2, x.fp[0], x.fm[0], x.z[0], a, b, d);
+        g_a += c.al * (re_dot(l.fp[4], a) + re_dot(l.fm[4], b) + re_dot(l.z[4], d));
+        rf3_adj(m1, c.zz, c.z2, l.fp[4], l.fm[4], l.z[4], sp, sm, sz);
+    }
+    *ga += B1 * g_a + g_al;
+    if constexpr (ND == 0) gth[3] += c.al * g_a;
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        float2 a, b, d;
+        rf3_adj(m, c.zz, c.z2, l.fp[j], l.fm[j], l.z[j], a, b, d);
+        l.fp[j] = a;
+        l.fm[j] = b;
+        l.z[j] = d;
+    }
+    if constexpr (ND > 0) {
+        l.fp[0] = c_add(l.fp[0], c_scale(sp, c.al));
+        l.fm[0] = c_add(l.fm[0], c_scale(sm, c.al));
+        l.z[0] = c_add(l.z[0], c_scale(sz, c.al));
+    }
+}
+
+// --- state storage (checkpoints / segment buffer), lanes contiguous -------
+
+template <int ND>
+__device__ __forceinline__ void st_store(float *base, int w, int k, const Aug<ND> &y)
+{
+    if (k >= w) return;
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        base[(6 * j + 0) * w + k] = y.fp[j].x;
+        base[(6 * j + 1) * w + k] = y.fp[j].y;
+        base[(6 * j + 2) * w + k] = y.fm[j].x;
+        base[(6 * j + 3) * w + k] = y.fm[j].y;
+        base[(6 * j + 4) * w + k] = y.z[j].x;
+        base[(6 * j + 5) * w + k] = y.z[j].y;
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ void st_load(const float *base, int w, int k, Aug<ND> &y)
+{
+#pragma unroll
+    for (int j = 0; j <= ND; j++) {
+        if (k < w) {
+            y.fp[j] = {base[(6 * j + 0) * w + k], base[(6 * j + 1) * w + k]};
+            y.fm[j] = {base[(6 * j + 2) * w + k], base[(6 * j + 3) * w + k]};
+            y.z[j] = {base[(6 * j + 4) * w + k], base[(6 * j + 5) * w + k]};
+        } else {
+            y.fp[j] = y.fm[j] = y.z[j] = {0.f, 0.f};
+        }
+    }
+}
+
+__device__ __forceinline__ Atom load_atom(const float *theta, size_t a)
+{
+    Atom p;
+    p.T1 = theta[4 * a];
+    p.T2 = theta[4 * a + 1];
+    p.B0 = theta[4 * a + 2];
+    p.B1 = theta[4 * a + 3];
+    p.iT1 = 1.f / p.T1;
+    p.iT2 = 1.f / p.T2;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian: s [n][T] and J [n][4][T] (either may be null)
+__global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T, int w,
+                                                          const float *theta, size_t n,
+                                                          float2 *sig, float2 *jac)
+{
+    const size_t a = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (a >= n) return;
+    const int k = threadIdx.x & 63;
+    const Atom p = load_atom(theta, a);
+    Aug<4> y;
+    f_init(y, p, k);
+    for (int t = 0; t < T; t++) {
+        const TrK c = tr_coef(seq[t], p);
+        Aug<4> r;
+        f_tr(y, c, p, k, w, &r);
+        if (k == 0) {
+            if (sig) sig[a * T + t] = r.fp[0];
+            if (jac)
+#pragma unroll
+                for (int d = 0; d < 4; d++) jac[(a * 4 + d) * T + t] = r.fp[1 + d];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweeps.  ND = 0: L = Re sum conj(v_n) s_n (adjoint).  ND = 4: the
+// CRB loss sum_d wts_d [F^-1]_dd / scale_d^2, F = Re(J^H J) (unit noise), whose
+// gradient needs the adjoint of the tangent recursion.
+struct RevArgs {
+    const DTr *seq;
+    int T, w, nck;
+    const float *theta;
+    size_t n;
+    const float2 *cot;         // ND = 0: [n][T]
+    float *grad_theta;         // ND = 0: [n][4] or null
+    float *grad_seq;           // [2][T] accumulated (FA raw units, TR) or null
+    float fa_unit;             // d alpha / d FA_raw
+    float wts[4];              // ND = 4
+    double *loss;              // ND = 4: [n]
+    float *scratch;            // per atom (nck + kCkpt) * 6 (1 + ND) * w floats
+};
+
+// Fisher matrix from lane 0's running sums; returns false if singular.
+__device__ bool inv4(const double *A, double *X)
+{
+    double m[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 8; c++) m[r][c] = c < 4 ? A[4 * r + c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int c = 0; c < 4; c++) {
+        int p = c;
+        for (int r = c + 1; r < 4; r++)
+            if (fabs(m[r][c]) > fabs(m[p][c])) p = r;
+        if (m[p][c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < 8; j++) {
+                const double t = m[c][j];
+                m[c][j] = m[p][j];
+                m[p][j] = t;
+            }
+        const double inv = 1.0 / m[c][c];
+        for (int j = 0; j < 8; j++) m[c][j] *= inv;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                const double f = m[r][c];
+                for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+            }
+    }
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) X[4 * r + c] = m[r][c + 4];
+    return true;
+}
+
+template <int ND>
+__global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
+{
+    const size_t a = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (a >= A.n) return;
+    const int k = threadIdx.x & 63;
+    const int w = A.w, T = A.T;
+    const int C = 6 * (1 + ND) * w;
+    float *ck = A.scratch + a * (size_t)(A.nck + kCkpt) * C;
+    float *seg = ck + (size_t)A.nck * C;
+    const Atom p = load_atom(A.theta, a);
+
+    // forward sweep: checkpoints every kCkpt TRs; Fisher sums (ND = 4)
+    Aug<ND> y;
+    f_init(y, p, k);
+    float F[10];
+#pragma unroll
+    for (int i = 0; i < 10; i++) F[i] = 0.f;
+    for (int t = 0; t < T; t++) {
+        if (t % kCkpt == 0) st_store(ck + (size_t)(t / kCkpt) * C, w, k, y);
+        const TrK c = tr_coef(A.seq[t], p);
+        if constexpr (ND > 0) {
+            Aug<ND> r;
+            f_tr(y, c, p, k, w, &r);
+            if (k == 0) {
+                int i = 0;
+#pragma unroll
+                for (int d = 0; d < 4; d++)
+#pragma unroll
+                    for (int e = d; e < 4; e++) F[i++] += re_dot(r.fp[1 + d], r.fp[1 + e]);
+            }
+        } else {
+            f_tr<ND>(y, c, p, k, w, nullptr);
+        }
+    }
+    // loss and dL/dF (ND = 4), on lane 0 in FP64, broadcast as G
+    float G[10];
+#pragma unroll
+    for (int i = 0; i < 10; i++) G[i] = 0.f;
+    if constexpr (ND > 0) {
+        if (k == 0) {
+            double Fd[16], Fi[16];
+            int i = 0;
+            for (int d = 0; d < 4; d++)
+                for (int e = d; e < 4; e++, i++) Fd[4 * d + e] = Fd[4 * e + d] = F[i];
+            double loss = 0.0;
+            if (inv4(Fd, Fi)) {
+                double Wd[4];
+                for (int d = 0; d < 4; d++) {
+                    const double sc = d == 0 ? p.T1 : (d == 1 ? p.T2 : 1.0);
+                    Wd[d] = A.wts[d] / (sc * sc);
+                    loss += Wd[d] * Fi[5 * d];
+                }
+                // dL/dF = -F^-1 W F^-1
+                i = 0;
+                for (int d = 0; d < 4; d++)
+                    for (int e = d; e < 4; e++, i++) {
+                        double g = 0.0;
+                        for (int m = 0; m < 4; m++) g -= Fi[4 * d + m] * Wd[m] * Fi[4 * m + e];
+                        G[i] = (float)g;
+                    }
+            } else {
+                loss = __builtin_nan("");
+            }
+            A.loss[a] = loss;
+        }
+#pragma unroll
+        for (int i = 0; i < 10; i++) G[i] = __shfl(G[i], 0);
+    }
+
+    // reverse sweep, segment by segment
+    Aug<ND> l;
+#pragma unroll
+    for (int j = 0; j <= ND; j++) l.fp[j] = l.fm[j] = l.z[j] = {0.f, 0.f};
+    float gth[4] = {0.f, 0.f, 0.f, 0.f};
+    const float beta = kCB0 * p.B0;
+    for (int sgi = A.nck - 1; sgi >= 0; sgi--) {
+        const int t0 = sgi * kCkpt;
+        const int t1 = min(T, t0 + kCkpt);
+        st_load(ck + (size_t)sgi * C, w, k, y);
+        for (int t = t0; t < t1; t++) {
+            st_store(seg + (size_t)(t - t0) * C, w, k, y);
+            f_tr<ND>(y, tr_coef(A.seq[t], p), p, k, w, nullptr);
+        }
+        for (int t = t1 - 1; t >= t0; t--) {
+            Aug<ND> yn;
+            st_load(seg + (size_t)(t - t0) * C, w, k, yn);
+            const TrK c = tr_coef(A.seq[t], p);
+            float ga = 0.f, gt = 0.f;
+            // relax(TR - TE): input = dephase(phase_b(phase_a(relax_a(rf(yn)))))
+            {
+                Aug<ND> x = yn;
+                f_rf(x, c);
+                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
+                f_phase(x, c.pa, c.te);
+                f_phase(x, c.pb, c.t2);
+                f_dephase(x, k, w);
+                r_relax(l, x, c.e1b, c.e2b, c.t2, p, k == 0, &gt, gth);
+            }
+            r_dephase(l, k, w);
+            // phase(TR - TE) and the readout: input = phase_a(relax_a(rf(yn)))
+            {
+                Aug<ND> x = yn;
+                f_rf(x, c);
+                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
+                f_phase(x, c.pa, c.te);
+                r_phase(l, x, c.pb, c.t2, beta, &gt, gth);
+                if (k == 0) {
+                    if constexpr (ND > 0) {
+                        // dL/dJ_e = 2 sum_d G_de J_d
+                        float2 J[4];
+#pragma unroll
+                        for (int d = 0; d < 4; d++) J[d] = x.fp[1 + d];
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            float2 cg = {0.f, 0.f};
+#pragma unroll
+                            for (int d = 0; d < 4; d++) {
+                                const int i = d <= e ? (d * 7 - d * d) / 2 + e : (e * 7 - e * e) / 2 + d;
+                                cg = c_add(cg, c_scale(J[d], 2.f * G[i]));
+                            }
+                            l.fp[1 + e] = c_add(l.fp[1 + e], cg);
+                        }
+                    } else {
+                        l.fp[0] = c_add(l.fp[0], A.cot[a * T + t]);
+                    }
+                }
+            }
+            // phase(TE): input = relax_a(rf(yn))
+            {
+                Aug<ND> x = yn;
+                f_rf(x, c);
+                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
+                r_phase(l, x, c.pa, c.te, beta, nullptr, gth);
+            }
+            // relax(TE): input = rf(yn)
+            {
+                Aug<ND> x = yn;
+                f_rf(x, c);
+                r_relax(l, x, c.e1a, c.e2a, c.te, p, k == 0, nullptr, gth);
+            }
+            r_rf(l, yn, c, p.B1, &ga, gth);
+            if (A.grad_seq) {
+                ga = wave_sum(ga);
+                gt = wave_sum(gt);
+                if (k == 0) {
+                    atomicAdd(&A.grad_seq[t], ga * A.fa_unit);
+                    atomicAdd(&A.grad_seq[T + t], gt);
+                }
+            }
+        }
+    }
+    if (ND == 0 && A.grad_theta) {
+        // init_rf_pulse's T1/T2 dependence (lane 0)
+        if (k == 0) {
+            const float a0 = (float)(180.0 * M_PI / 180.0f);
+            const float sa0 = sinf(a0), ca0 = cosf(a0);
+            const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
+            const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
+            gth[0] += l.z[0].x * e1d * (ca0 - 1.f);
+            const float2 t2 = {0.f, e2d * sa0};
+            gth[1] += re_dot(l.fp[0], c_conj(t2)) + re_dot(l.fm[0], t2);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) gth[d] = wave_sum(gth[d]);
+        if (k == 0)
+#pragma unroll
+            for (int d = 0; d < 4; d++) A.grad_theta[a * 4 + d] = gth[d];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Newton (Levenberg-Marquardt damped) refinement of (T1, T2, B0, B1)
+// per voxel with the complex proton density eliminated (variable
+// projection): cost c = |x|^2 - |s^H x|^2 / |s|^2.  A step is kept only if
+// it lowers c, so the cost history is monotone.
+struct GnArgs {
+    const DTr *seq;
+    int T, w, iters;
+    const float2 *sig;         // [V][T]
+    size_t V;
+    float *theta;              // [V][4] in/out
+    float *pd;                 // [V] or null: |s^H x| / |s|
+    float *cost;               // [V][iters + 1] or null
+};
+
+// lane 0's sums of one pass: S = s^H s, Sx = s^H x, U_d = s^H ds_d,
+// W_d = ds_d^H x, D_de = ds_d^H ds_e (d <= e)
+struct GnSums {
+    float S;
+    float2 Sx, U[4], W[4], D[10];
+};
+
+template <int ND>
+__device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const float2 *x, int k)
+{
+    GnSums g{};
+    Aug<ND> y;
+    f_init(y, p, k);
+    for (int t = 0; t < T; t++) {
+        Aug<ND> r;
+        f_tr(y, tr_coef(seq[t], p), p, k, w, &r);
+        if (k == 0) {
+            const float2 s = r.fp[0], xv = x[t];
+            g.S += re_dot(s, s);
+            g.Sx = c_add(g.Sx, c_mul(c_conj(s), xv));
+            if constexpr (ND > 0) {
+                int i = 0;
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    const float2 ds = r.fp[1 + d];
+                    g.U[d] = c_add(g.U[d], c_mul(c_conj(s), ds));
+                    g.W[d] = c_add(g.W[d], c_mul(c_conj(ds), xv));
+#pragma unroll
+                    for (int e = d; e < 4; e++, i++) g.D[i] = c_add(g.D[i], c_mul(c_conj(ds), r.fp[1 + e]));
+                }
+            }
+        }
+    }
+    return g;
+}
+
+__device__ __forceinline__ bool theta_ok(const float *th)
+{
+    return th[0] > 0.f && th[1] > 0.f && th[3] > 0.f && isfinite(th[0]) && isfinite(th[1]) &&
+           isfinite(th[2]) && isfinite(th[3]);
+}
+
+__device__ __forceinline__ Atom make_atom(const float *th)
+{
+    Atom p;
+    p.T1 = th[0];
+    p.T2 = th[1];
+    p.B0 = th[2];
+    p.B1 = th[3];
+    p.iT1 = 1.f / p.T1;
+    p.iT2 = 1.f / p.T2;
+    return p;
+}
+
+__global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
+{
+    const size_t v = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (v >= A.V) return;
+    const int k = threadIdx.x & 63;
+    const float2 *x = A.sig + v * A.T;
+    float xx = 0.f;
+    for (int t = k; t < A.T; t += 64) xx += re_dot(x[t], x[t]);
+    xx = wave_sum(xx);
+    float th[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
+    GnSums g = gn_pass<4>(A.seq, A.T, A.w, make_atom(th), x, k);
+    float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
+    c = __shfl(c, 0);
+    if (k == 0 && A.cost) A.cost[v * (A.iters + 1)] = c;
+    double lam = 1e-3;
+    for (int it = 0; it < A.iters; it++) {
+        float trial[4] = {0.f, 0.f, 0.f, 0.f};
+        if (k == 0) {
+            // normal equations in FP64
+            const double S = g.S;
+            const double rr = g.Sx.x / S, ri = g.Sx.y / S;     // rho = Sx / S
+            const double r2 = rr * rr + ri * ri;
+            double Am[16], b[4];
+            int i = 0;
+            for (int d = 0; d < 4; d++)
+                for (int e = d; e < 4; e++, i++) {
+                    // Re[D_de - conj(U_d) U_e / S]
+                    const double cu = (double)g.U[d].x * g.U[e].x + (double)g.U[d].y * g.U[e].y;
+                    Am[4 * d + e] = Am[4 * e + d] = r2 * ((double)g.D[i].x - cu / S);
+                }
+            for (int d = 0; d < 4; d++) {
+                // g_d = -Re[conj(rho) (W_d - rho conj(U_d))]
+                const double wr = g.W[d].x - (rr * g.U[d].x + ri * g.U[d].y);
+                const double wi = g.W[d].y - (ri * g.U[d].x - rr * g.U[d].y);
+                b[d] = rr * wr + ri * wi;   // = -g_d
+            }
+            double Md[16], Mi[16];
+            for (int q = 0; q < 16; q++) Md[q] = Am[q];
+            for (int d = 0; d < 4; d++) Md[5 * d] = Am[5 * d] * (1.0 + lam);
+            double dl[4] = {0, 0, 0, 0};
+            if (inv4(Md, Mi))
+                for (int d = 0; d < 4; d++)
+                    for (int e = 0; e < 4; e++) dl[d] += Mi[4 * d + e] * b[e];
+            for (int d = 0; d < 4; d++) trial[d] = (float)(th[d] + dl[d]);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) trial[d] = __shfl(trial[d], 0);
+        float ct = INFINITY;
+        if (theta_ok(trial)) {
+            const GnSums h = gn_pass<0>(A.seq, A.T, A.w, make_atom(trial), x, k);
+            ct = xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
+            ct = __shfl(ct, 0);
+        }
+        if (ct < c) {
+#pragma unroll
+            for (int d = 0; d < 4; d++) th[d] = trial[d];
+            c = ct;
+            lam = fmax(lam * 0.1, 1e-9);
+            if (it + 1 < A.iters) g = gn_pass<4>(A.seq, A.T, A.w, make_atom(th), x, k);
+            else g = gn_pass<0>(A.seq, A.T, A.w, make_atom(th), x, k);
+        } else {
+            lam = fmin(lam * 10.0, 1e12);
+        }
+        if (k == 0 && A.cost) A.cost[v * (A.iters + 1) + it + 1] = c;
+    }
+    if (k == 0) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) A.theta[v * 4 + d] = th[d];
+        if (A.pd) A.pd[v] = sqrtf(g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / sqrtf(g.S);
+    }
+}
+
+}  // namespace
+
+size_t diff_scratch_floats(int nd, int T, int w)
+{
+    const int nck = (T + kCkpt - 1) / kCkpt;
+    return (size_t)(nck + kCkpt) * 6 * (1 + nd) * w;
+}
+
+int launch_jacobian(const DTr *seq, int T, int w, const float *theta, size_t n, float2 *sig,
+                    float2 *jac, hipStream_t s)
+{
+    if (n == 0) return 0;
+    jac_kernel<<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(seq, T, w, theta,
+                                                                                  n, sig, jac);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "jacobian launch: %s", hipGetErrorString(e));
+}
+
+int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, const float2 *cot,
+                   float *grad_theta, float *grad_seq, float fa_unit, float *scratch, hipStream_t s)
+{
+    if (n == 0) return 0;
+    RevArgs A{};
+    A.seq = seq;
+    A.T = T;
+    A.w = w;
+    A.nck = (T + kCkpt - 1) / kCkpt;
+    A.theta = theta;
+    A.n = n;
+    A.cot = cot;
+    A.grad_theta = grad_theta;
+    A.grad_seq = grad_seq;
+    A.fa_unit = fa_unit;
+    A.scratch = scratch;
+    rev_kernel<0><<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "adjoint launch: %s", hipGetErrorString(e));
+}
+
+int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const float *wts,
+               double *loss, float *grad_seq, float fa_unit, float *scratch, hipStream_t s)
+{
+    if (n == 0) return 0;
+    RevArgs A{};
+    A.seq = seq;
+    A.T = T;
+    A.w = w;
+    A.nck = (T + kCkpt - 1) / kCkpt;
+    A.theta = theta;
+    A.n = n;
+    A.grad_seq = grad_seq;
+    A.fa_unit = fa_unit;
+    for (int d = 0; d < 4; d++) A.wts[d] = wts[d];
+    A.loss = loss;
+    A.scratch = scratch;
+    rev_kernel<4><<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "crb launch: %s", hipGetErrorString(e));
+}
+
+int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
+              float *pd, float *cost, hipStream_t s)
+{
+    if (V == 0) return 0;
+    GnArgs A{};
+    A.seq = seq;
+    A.T = T;
+    A.w = w;
+    A.iters = iters;
+    A.sig = sig;
+    A.V = V;
+    A.theta = theta;
+    A.pd = pd;
+    A.cost = cost;
+    gn_kernel<<<dim3((unsigned)((V + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "gn launch: %s", hipGetErrorString(e));
+}
+
+}  // namespace mrf

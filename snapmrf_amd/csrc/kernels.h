@@ -76,6 +76,27 @@ inline size_t lowp_ld(size_t T) { return (T + 31) / 32 * 32; }
 int launch_lowp_prepare(const float2 *src, const int32_t *perm, size_t rows, int T, int ld,
                         uint32_t *hi, uint32_t *lo, hipStream_t s);
 
+// Differentiation form (grad.hip) --------------------------------------
+
+// per-TR table of the continuous EPG-SSFP model (unit transform in double)
+struct DTr {
+    float al;        // flip angle (rad) before B1
+    float cph, sph;  // e^{i phi}
+    float c2p, s2p;  // e^{2 i phi}
+    float te, t2;    // TE and TR - TE (ms)
+    float pad;
+};
+// scratch floats per atom of launch_adjoint (nd = 0) / launch_crb (nd = 4)
+size_t diff_scratch_floats(int nd, int T, int w);
+int launch_jacobian(const DTr *seq, int T, int w, const float *theta, size_t n, float2 *sig,
+                    float2 *jac, hipStream_t s);
+int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, const float2 *cot,
+                   float *grad_theta, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
+int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const float *wts,
+               double *loss, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
+int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
+              float *pd, float *cost, hipStream_t s);
+
 // B1 slice per voxel: argmin_k |b1map[v] - slice_b1[k]| (strict <, init 10).
 int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,
                      size_t V, int32_t *vox_slice, hipStream_t s);

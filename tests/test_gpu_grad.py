@@ -1,0 +1,249 @@
+"""Differentiation form on the GPU through the C ABI: Jacobians, adjoints
+(checkpointed every 50 TRs), Gauss-Newton refinement and the CRB design-loss
+gradient, against the FP64 oracle (oracle/diff.c, itself pinned by
+tests/test_oracle_diff.py) and FP64 central finite differences.  Tolerance:
+1e-4 relative (FP32 recursion over up to 1000 TRs)."""
+import numpy as np
+import pytest
+
+import oracle
+import snapmrf_amd as sm
+from snapmrf_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+THETAS = np.array([[900, 80, 7, 0.93], [300, 250, -31, 1.12], [2500, 30, 0, 0.85],
+                   [1200, 60, 18.5, 1.0], [150, 40, -2, 0.8], [4000, 1500, 33, 1.2]], np.float32)
+
+
+def _raw(T, seed=3):
+    return synth.csv_columns(synth.sequence(T, seed=seed), "epg_ssfp").astype(np.float32)
+
+
+def _jac(torch, raw, theta, w):
+    n, T = len(theta), raw.shape[1]
+    d_th = torch.from_numpy(np.ascontiguousarray(theta, np.float32)).cuda()
+    s = torch.empty((n, T), dtype=torch.complex64, device="cuda")
+    J = torch.empty((n, 4, T), dtype=torch.complex64, device="cuda")
+    sm.epg_jacobian(raw, d_th, n, w, d_sig=s, d_jac=J)
+    torch.cuda.synchronize()
+    return s.cpu().numpy(), J.cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("T,w", [(137, 17), (1000, 30), (60, 2), (250, 64)])
+def test_jacobian_matches_oracle(gpu, T, w):
+    raw = _raw(T)
+    s, J = _jac(gpu, raw, THETAS, w)
+    for a, th in enumerate(THETAS):
+        s64, J64 = oracle.dm_jacobian(raw, th.astype(np.float64), w)
+        assert _rel(s[a], s64) < TOL, a
+        for d in range(4):
+            assert _rel(J[a, d], J64[d]) < TOL, (a, d)
+
+
+def test_jacobian_central_differences(gpu):
+    """GPU dS/dtheta against FP64 central differences (h = 1e-4 relative) of
+    the oracle forward, T = 1000."""
+    raw = _raw(1000, seed=8)
+    w = 30
+    _, J = _jac(gpu, raw, THETAS[:3], w)
+    for a, th in enumerate(THETAS[:3].astype(np.float64)):
+        for d in range(4):
+            h = 1e-4 * (abs(th[d]) if th[d] != 0 else 1.0)
+            tp, tm = th.copy(), th.copy()
+            tp[d] += h
+            tm[d] -= h
+            fd = (oracle.dm_jacobian(raw, tp, w, False) - oracle.dm_jacobian(raw, tm, w, False)) / (2 * h)
+            assert _rel(J[a, d], fd) < TOL, (a, d)
+
+
+@pytest.mark.parametrize("T,w", [(137, 17), (50, 8), (49, 8), (1000, 30)])
+def test_adjoint_matches_oracle(gpu, T, w):
+    """Theta and sequence gradients of L = sum_atoms Re(v^H s), with
+    checkpoint boundaries at 50, 100 (T = 137), exactly one segment (50) and
+    a partial single segment (49)."""
+    raw = _raw(T, seed=4)
+    n = len(THETAS)
+    rng = np.random.default_rng(T)
+    v = (rng.normal(size=(n, T)) + 1j * rng.normal(size=(n, T))).astype(np.complex64)
+    d_th = gpu.from_numpy(THETAS).cuda()
+    d_v = gpu.from_numpy(v).cuda()
+    g_th = gpu.empty((n, 4), device="cuda")
+    g_seq = gpu.zeros((2, T), device="cuda")
+    sm.epg_adjoint(raw, d_th, n, w, d_v, d_grad_theta=g_th, d_grad_seq=g_seq)
+    gpu.cuda.synchronize()
+    g_th, g_seq = g_th.cpu().numpy(), g_seq.cpu().numpy()
+    sfa = np.zeros(T)
+    str_ = np.zeros(T)
+    for a, th in enumerate(THETAS.astype(np.float64)):
+        g, gfa, gtr = oracle.dm_adjoint(raw, th, w, v[a].astype(np.complex128))
+        assert np.all(np.abs(g_th[a] - g) <= TOL * np.abs(g) + 1e-6 * np.abs(g).max()), (a, g_th[a], g)
+        sfa += gfa
+        str_ += gtr
+    assert np.abs(g_seq[0] - sfa).max() <= TOL * np.abs(sfa).max()
+    assert np.abs(g_seq[1] - str_).max() <= TOL * np.abs(str_).max()
+
+
+def test_adjoint_dot_product_on_gpu(gpu):
+    """<J u, v> = <u, J^T v> between the GPU's own forward and reverse modes."""
+    T, w = 300, 24
+    raw = _raw(T, seed=9)
+    n = len(THETAS)
+    _, J = _jac(gpu, raw, THETAS, w)
+    rng = np.random.default_rng(5)
+    v = (rng.normal(size=(n, T)) + 1j * rng.normal(size=(n, T))).astype(np.complex64)
+    g_th = gpu.empty((n, 4), device="cuda")
+    sm.epg_adjoint(raw, gpu.from_numpy(THETAS).cuda(), n, w, gpu.from_numpy(v).cuda(),
+                   d_grad_theta=g_th)
+    gpu.cuda.synchronize()
+    g_th = g_th.cpu().numpy()
+    for a in range(n):
+        want = np.array([np.real(np.vdot(v[a].astype(np.complex128), J[a, d])) for d in range(4)])
+        assert np.all(np.abs(g_th[a] - want) <= TOL * np.abs(want).max()), a
+
+
+def test_adjoint_flip_angle_central_differences(gpu):
+    """dL/dFA_raw and dL/dTR_raw against FP64 central differences of the
+    oracle forward on 32 entries, T = 1000, w = 30."""
+    T, w = 1000, 30
+    raw = _raw(T, seed=11)
+    th = THETAS[0].astype(np.float64)
+    rng = np.random.default_rng(3)
+    v = (rng.normal(size=T) + 1j * rng.normal(size=T)).astype(np.complex64)
+    g_seq = gpu.zeros((2, T), device="cuda")
+    sm.epg_adjoint(raw, gpu.from_numpy(THETAS[:1]).cuda(), 1, w, gpu.from_numpy(v[None]).cuda(),
+                   d_grad_seq=g_seq)
+    gpu.cuda.synchronize()
+    g_seq = g_seq.cpu().numpy()
+    vd = v.astype(np.complex128)
+
+    def loss(r):
+        return float(np.real(np.vdot(vd, oracle.dm_jacobian(r, th, w, False))))
+
+    for col, row in ((0, 0), (2, 1)):
+        scale = np.abs(g_seq[row]).max()
+        for n in np.linspace(0, T - 2, 16).astype(int):
+            r1, r2 = raw.copy(), raw.copy()
+            h = 2e-3 * max(1.0, abs(float(raw[col, n])))
+            r1[col, n] = np.float32(raw[col, n] + h)
+            r2[col, n] = np.float32(raw[col, n] - h)
+            fd = (loss(r1) - loss(r2)) / (float(r1[col, n]) - float(r2[col, n]))
+            assert abs(g_seq[row, n] - fd) <= TOL * scale, (col, n)
+
+
+def test_gauss_newton_recovers_off_grid_truths(gpu):
+    """Noise-free signals of off-grid truths: from a start one grid step away
+    the refinement drives the cost to ~0 and recovers (T1, T2, B0, B1); the
+    cost history never increases."""
+    T, w, iters = 400, 24, 25
+    raw = _raw(T, seed=6)
+    rng = np.random.default_rng(7)
+    V = 64
+    truth = np.stack([rng.uniform(300, 2500, V), rng.uniform(30, 300, V),
+                      rng.uniform(-30, 30, V), rng.uniform(0.85, 1.15, V)], 1).astype(np.float32)
+    s, _ = _jac(gpu, raw, truth, w)
+    rho = (rng.uniform(0.5, 2.0, V) * np.exp(1j * rng.uniform(0, 2 * np.pi, V))).astype(np.complex64)
+    x = (s * rho[:, None]).astype(np.complex64)
+    start = truth * np.array([1.06, 0.94, 1.0, 1.0], np.float32) + np.array([0, 0, 1.5, 0.04], np.float32)
+    d_th = gpu.from_numpy(start.copy()).cuda()
+    d_pd = gpu.empty(V, device="cuda")
+    d_cost = gpu.empty((V, iters + 1), device="cuda")
+    sm.gn_refine(raw, gpu.from_numpy(x).cuda(), V, w, iters, d_th, d_pd=d_pd, d_cost=d_cost)
+    gpu.cuda.synchronize()
+    th, pd, cost = d_th.cpu().numpy(), d_pd.cpu().numpy(), d_cost.cpu().numpy()
+    assert np.all(np.diff(cost, axis=1) <= 0.0)
+    xx = np.sum(np.abs(x) ** 2, axis=1)
+    assert np.all(cost[:, -1] <= 1e-6 * xx)
+    assert np.all(cost[:, 0] > 100 * cost[:, -1])
+    err = np.abs(th - truth)
+    assert np.all(err[:, 0] <= 1e-3 * truth[:, 0])
+    assert np.all(err[:, 1] <= 1e-3 * truth[:, 1])
+    assert np.all(err[:, 2] <= 1e-2)
+    assert np.all(err[:, 3] <= 1e-3)
+    want_pd = np.abs(rho) * np.linalg.norm(s, axis=1)
+    assert np.allclose(pd, want_pd, rtol=1e-3)
+
+
+def test_crb_loss_and_gradient(gpu):
+    """CRB loss per tissue against the FP64 oracle, and its flip-angle and TR
+    gradients against FP64 central differences of the oracle loss on 32
+    entries (64 tissues, w = 60, T = 300)."""
+    T, w = 300, 60
+    raw = _raw(T, seed=12)
+    rng = np.random.default_rng(9)
+    n = 64
+    tis = np.stack([rng.uniform(300, 2500, n), rng.uniform(30, 300, n), rng.uniform(-30, 30, n),
+                    rng.uniform(0.85, 1.15, n)], 1).astype(np.float32)
+    wts = np.array([1.0, 1.0, 0.01, 0.1], np.float32)
+    d_loss = gpu.empty(n, dtype=gpu.float64, device="cuda")
+    g_seq = gpu.zeros((2, T), device="cuda")
+    sm.crb_loss_grad(raw, gpu.from_numpy(tis).cuda(), n, w, wts, d_loss, d_grad_seq=g_seq)
+    gpu.cuda.synchronize()
+    loss, g_seq = d_loss.cpu().numpy(), g_seq.cpu().numpy()
+    t64 = tis.astype(np.float64)
+    w64 = wts.astype(np.float64)
+    for i in range(0, n, 9):
+        want = oracle.dm_crb(raw, t64[i:i + 1], w, w64)
+        assert abs(loss[i] - want) <= 1e-3 * want, i
+    total = oracle.dm_crb(raw, t64, w, w64)
+    assert abs(loss.sum() - total) <= 1e-3 * total
+    sub = t64[:8]   # FD over a tissue subset keeps the CPU cost small
+    d_loss8 = gpu.empty(8, dtype=gpu.float64, device="cuda")
+    g8 = gpu.zeros((2, T), device="cuda")
+    sm.crb_loss_grad(raw, gpu.from_numpy(tis[:8]).cuda(), 8, w, wts, d_loss8, d_grad_seq=g8)
+    gpu.cuda.synchronize()
+    g8 = g8.cpu().numpy()
+    for col, row, idx in ((0, 0, np.linspace(0, T - 1, 24).astype(int)),
+                          (2, 1, np.linspace(0, T - 2, 8).astype(int))):
+        scale = np.abs(g8[row]).max()
+        for t in idx:
+            r1, r2 = raw.copy(), raw.copy()
+            h = 1e-3 * max(1.0, abs(float(raw[col, t])))
+            r1[col, t] = np.float32(raw[col, t] + h)
+            r2[col, t] = np.float32(raw[col, t] - h)
+            fd = (oracle.dm_crb(r1, sub, w, w64) - oracle.dm_crb(r2, sub, w, w64)) / (
+                float(r1[col, t]) - float(r2[col, t]))
+            assert abs(g8[row, t] - fd) <= 1e-3 * scale, (col, t, g8[row, t], fd)
+
+
+def test_autograd_function(gpu):
+    """torch.autograd binding: forward = the C-ABI signal, backward = the
+    adjoint, checked against the FP64 oracle adjoint."""
+    from snapmrf_amd.autograd import epg_ssfp_signal
+    torch = gpu
+    T, w = 120, 20
+    raw = _raw(T, seed=2)
+    theta = torch.from_numpy(THETAS[:3].copy()).cuda().requires_grad_(True)
+    fa = torch.from_numpy(raw[0].copy()).requires_grad_(True)
+    ph = torch.from_numpy(raw[1].copy())
+    tr = torch.from_numpy(raw[2].copy()).requires_grad_(True)
+    te = torch.from_numpy(raw[3].copy())
+    s = epg_ssfp_signal(theta, fa, tr, ph, te, w)
+    rng = np.random.default_rng(1)
+    v = (rng.normal(size=(3, T)) + 1j * rng.normal(size=(3, T))).astype(np.complex64)
+    L = (torch.conj(torch.from_numpy(v).cuda()) * s).real.sum()
+    L.backward()
+    gfa_sum = np.zeros(T)
+    for a, th in enumerate(THETAS[:3].astype(np.float64)):
+        g, gfa, gtr = oracle.dm_adjoint(raw, th, w, v[a].astype(np.complex128))
+        gt = theta.grad[a].cpu().numpy()
+        assert np.all(np.abs(gt - g) <= TOL * np.abs(g) + 1e-6 * np.abs(g).max())
+        gfa_sum += gfa
+    assert np.abs(fa.grad.numpy() - gfa_sum).max() <= TOL * np.abs(gfa_sum).max()
+    assert fa.grad.device.type == "cpu" and tr.grad is not None
+
+
+def test_bad_arguments(gpu):
+    raw = _raw(10)
+    d_th = gpu.from_numpy(THETAS[:1]).cuda()
+    with pytest.raises(RuntimeError, match="nstates"):
+        sm.epg_jacobian(raw, d_th, 1, 1, d_sig=gpu.empty((1, 10), dtype=gpu.complex64, device="cuda"))
+    with pytest.raises(RuntimeError, match="nstates"):
+        sm.epg_jacobian(raw, d_th, 1, 65, d_sig=gpu.empty((1, 10), dtype=gpu.complex64, device="cuda"))
+    with pytest.raises(RuntimeError, match="iters"):
+        sm.gn_refine(raw, gpu.empty((1, 10), dtype=gpu.complex64, device="cuda"), 1, 8, -1, d_th)
