@@ -246,6 +246,113 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     return res
 
 
+# Algorithmic FP32 flops per (atom, EPG state, TR) of the differentiation
+# kernels: the Jacobian propagates the 102-flop SSFP step for the signal and 4
+# tangents plus their source terms (B1: 78, T1/T2: 24, B0: 40); the adjoint
+# counts one forward, one adjoint and one parameter contraction step.
+FLOP_JAC = 5 * 102 + 78 + 24 + 40
+FLOP_ADJ = 3 * 102
+
+
+def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, natoms, stream,
+                    timed):
+    """Jacobian, adjoint and Gauss-Newton over 240x240 off-grid voxels, and
+    the CRB design-loss gradient over 64 tissues (w = 60)."""
+    V = 240 * 240
+    rng = np.random.default_rng(21)
+    truth = np.stack([rng.uniform(300, 2500, V), rng.uniform(30, 300, V),
+                      rng.uniform(-30, 30, V), rng.uniform(0.85, 1.15, V)], 1).astype(np.float32)
+    d_truth = torch.from_numpy(truth).cuda()
+    d_s = torch.empty((V, T), dtype=torch.complex64, device="cuda")
+    d_J = torch.empty((V, 4, T), dtype=torch.complex64, device="cuda")
+    run_jac = lambda: sm.epg_jacobian(raw, d_truth, V, w, d_sig=d_s, d_jac=d_J, stream=stream)
+    run_jac()
+    jac_ms = timed(run_jac, 1)
+    del d_J
+    # noisy off-grid signals (SNR 25 on unit-norm atoms, PD ~ U(0.3, 1))
+    s_hat = d_s / torch.linalg.vector_norm(d_s, dim=1, keepdim=True)
+    pd = torch.from_numpy(rng.uniform(0.3, 1.0, V).astype(np.float32)).cuda()
+    ph = torch.from_numpy(rng.uniform(0, 2 * np.pi, V).astype(np.float32)).cuda()
+    sigma = 1.0 / (25.0 * np.sqrt(T))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(22)
+    noise = torch.complex(torch.randn((V, T), generator=gen, device="cuda"),
+                          torch.randn((V, T), generator=gen, device="cuda")) * (sigma / np.sqrt(2))
+    d_x = (s_hat * (pd * torch.exp(1j * ph))[:, None] + noise).to(torch.complex64).contiguous()
+    del s_hat, noise
+    d_maps = torch.empty((V, 5), device="cuda")
+    d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+    sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T, A, natoms, d_x, V, d_maps, d_idx=d_idx,
+                stream=stream)
+    torch.cuda.synchronize()
+    start = params[d_idx.cpu().numpy()]
+    iters = 10
+    d_th = torch.from_numpy(start.copy()).cuda()
+    d_cost = torch.empty((V, iters + 1), device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    sm.gn_refine(raw, d_x, V, w, iters, d_th, d_cost=d_cost, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gn_ms = e0.elapsed_time(e1)
+    ref = d_th.cpu().numpy()
+    cost = d_cost.cpu().numpy()
+
+    def errs(est):
+        return {"T1_median_rel": float(np.median(np.abs(est[:, 0] - truth[:, 0]) / truth[:, 0])),
+                "T2_median_rel": float(np.median(np.abs(est[:, 1] - truth[:, 1]) / truth[:, 1])),
+                "B0_median_abs_hz": float(np.median(np.abs(est[:, 2] - truth[:, 2]))),
+                "B1_median_abs": float(np.median(np.abs(est[:, 3] - truth[:, 3])))}
+    # adjoint of a random cotangent over the same voxels
+    d_cot = torch.complex(torch.randn((V, T), generator=gen, device="cuda"),
+                          torch.randn((V, T), generator=gen, device="cuda")).contiguous()
+    g_th = torch.empty((V, 4), device="cuda")
+    g_seq = torch.zeros((2, T), device="cuda")
+    run_adj = lambda: sm.epg_adjoint(raw, d_truth, V, w, d_cot, d_grad_theta=g_th,
+                                     d_grad_seq=g_seq, stream=stream)
+    run_adj()
+    adj_ms = timed(run_adj, 1)
+    del d_cot, d_x, d_s
+    # CRB design loss over 64 tissues, w = 60
+    nt, wc = 64, 60
+    tis = np.stack([rng.uniform(300, 2500, nt), rng.uniform(30, 300, nt), rng.uniform(-30, 30, nt),
+                    rng.uniform(0.85, 1.15, nt)], 1).astype(np.float32)
+    d_tis = torch.from_numpy(tis).cuda()
+    d_loss = torch.empty(nt, dtype=torch.float64, device="cuda")
+    g_crb = torch.zeros((2, T), device="cuda")
+    run_crb = lambda: sm.crb_loss_grad(raw, d_tis, nt, wc, [1.0, 1.0, 0.0, 0.0], d_loss,
+                                       d_grad_seq=g_crb, stream=stream)
+    run_crb()
+    crb_ms = timed(run_crb, 1)
+    jf = float(V) * T * w * FLOP_JAC
+    af = float(V) * T * w * FLOP_ADJ
+    res = {
+        "workload": f"240x240 = {V} off-grid voxels (T1 U(300,2500), T2 U(30,300), B0 U(-30,30), "
+                    f"B1 U(0.85,1.15)), T={T}, w={w}; CRB over 64 tissues, w=60",
+        "jacobian": {"ms": jac_ms, "atoms_per_s": V / (jac_ms * 1e-3),
+                     "tflops_algorithmic": jf / (jac_ms * 1e-3) / 1e12,
+                     "frac_fp32_peak": jf / (jac_ms * 1e-3) / 1e12 / PEAK_FP32,
+                     "flop_per_state_tr": FLOP_JAC},
+        "adjoint": {"ms": adj_ms, "atoms_per_s": V / (adj_ms * 1e-3),
+                    "tflops_algorithmic": af / (adj_ms * 1e-3) / 1e12,
+                    "frac_fp32_peak": af / (adj_ms * 1e-3) / 1e12 / PEAK_FP32,
+                    "flop_per_state_tr": FLOP_ADJ, "checkpoint_every": 50},
+        "gauss_newton": {"ms": gn_ms, "iters": iters, "voxels_per_s": V / (gn_ms * 1e-3),
+                         "start_from": "FP32 grid match (all B1 slices)",
+                         "errors_grid_match": errs(start), "errors_refined": errs(ref),
+                         "cost_monotone": bool(np.all(np.diff(cost, axis=1) <= 0)),
+                         "median_cost_ratio": float(np.median(cost[:, -1] / cost[:, 0]))},
+        "crb": {"ms": crb_ms, "tissues": nt, "w": wc,
+                "loss": float(d_loss.sum().item())},
+    }
+    log(f"differentiation: jacobian {jac_ms:.1f} ms, adjoint {adj_ms:.1f} ms, "
+        f"GN x{iters} {gn_ms:.1f} ms, CRB {crb_ms:.1f} ms")
+    del d_truth, g_th, g_seq
+    torch.cuda.empty_cache()
+    return res
+
+
 def measured_traffic(key: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
     summary (profiles/traffic.json, FETCH_SIZE + WRITE_SIZE scaled by the
@@ -267,6 +374,7 @@ def main() -> int:
     ap.add_argument("--small", action="store_true", help="reduced grid (debugging only)")
     ap.add_argument("--skip-match", action="store_true")
     ap.add_argument("--skip-lowp", action="store_true", help="skip the BF16 precision form")
+    ap.add_argument("--skip-diff", action="store_true", help="skip the differentiation form")
     ap.add_argument("--oracle-voxels", type=int, default=256,
                     help="voxels of the precision form checked against the FP64 oracle")
     ap.add_argument("--profile", action="store_true",
@@ -376,6 +484,9 @@ def main() -> int:
                                  "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
                                  "algorithmic_flops_per_launch": mflops}
+    if not args.skip_diff:
+        out["differentiation"] = differentiation(sm, torch, np, raw, T, w, d_atoms, d_params,
+                                                 params, A, natoms, stream, timed)
     if not args.skip_lowp:
         if not args.skip_match:
             del d_sig, d_b1, d_maps, d_idx
