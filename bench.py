@@ -255,7 +255,7 @@ FLOP_ADJ = 3 * 102
 
 
 def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, natoms, stream,
-                    timed):
+                    timed, fd_atoms=1024):
     """Jacobian, adjoint and Gauss-Newton over 240x240 off-grid voxels, and
     the CRB design-loss gradient over 64 tissues (w = 60)."""
     V = 240 * 240
@@ -268,6 +268,33 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
     run_jac = lambda: sm.epg_jacobian(raw, d_truth, V, w, d_sig=d_s, d_jac=d_J, stream=stream)
     run_jac()
     jac_ms = timed(run_jac, 1)
+    # Jacobian against FP64 central differences (h = 1e-4 relative) of the
+    # oracle forward on 1,024 atoms
+    fd_err = None
+    if fd_atoms:
+        import concurrent.futures as cf
+        import oracle
+        sel = np.sort(np.random.default_rng(23).choice(V, fd_atoms, replace=False))
+        Jg = d_J[torch.from_numpy(sel).cuda()].cpu().numpy()
+
+        def one(i):
+            th = truth[sel[i]].astype(np.float64)
+            worst = 0.0
+            for d in range(4):
+                h = 1e-4 * abs(th[d]) if th[d] != 0 else 1e-4
+                tp, tm = th.copy(), th.copy()
+                tp[d] += h
+                tm[d] -= h
+                fd = (oracle.dm_jacobian(raw, tp, w, False) - oracle.dm_jacobian(raw, tm, w, False)) / (2 * h)
+                worst = max(worst, float(np.abs(Jg[i, d] - fd).max() / np.abs(fd).max()))
+            return worst
+        t0 = time.time()
+        with cf.ThreadPoolExecutor(max_workers=16) as ex:
+            errs_fd = list(ex.map(one, range(fd_atoms)))
+        fd_err = {"atoms": fd_atoms, "max_rel_err": float(max(errs_fd)),
+                  "median_rel_err": float(np.median(errs_fd)), "h_rel": 1e-4}
+        log(f"Jacobian FD check on {fd_atoms} atoms: max rel err {max(errs_fd):.2e} "
+            f"({time.time() - t0:.1f} s)")
     del d_J
     # noisy off-grid signals (SNR 25 on unit-norm atoms, PD ~ U(0.3, 1))
     s_hat = d_s / torch.linalg.vector_norm(d_s, dim=1, keepdim=True)
@@ -278,7 +305,7 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
     gen.manual_seed(22)
     noise = torch.complex(torch.randn((V, T), generator=gen, device="cuda"),
                           torch.randn((V, T), generator=gen, device="cuda")) * (sigma / np.sqrt(2))
-    d_x = (s_hat * (pd * torch.exp(1j * ph))[:, None] + noise).to(torch.complex64).contiguous()
+    d_x = (s_hat * (pd * torch.exp(1j * ph))[:, None] + noise * pd[:, None]).to(torch.complex64).contiguous()
     del s_hat, noise
     d_maps = torch.empty((V, 5), device="cuda")
     d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
@@ -325,6 +352,34 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
                                        d_grad_seq=g_crb, stream=stream)
     run_crb()
     crb_ms = timed(run_crb, 1)
+    crb_fd = None
+    if fd_atoms:
+        # gradient of the summed loss w.r.t. 32 random flip-angle entries
+        # against FP64 central differences of the oracle loss
+        import concurrent.futures as cf
+        import oracle
+        g_crb.zero_()
+        run_crb()
+        torch.cuda.synchronize()
+        gfa = g_crb[0].cpu().numpy()
+        ent = np.sort(np.random.default_rng(24).choice(T, 32, replace=False))
+        t64 = tis.astype(np.float64)
+        w64 = np.array([1.0, 1.0, 0.0, 0.0])
+
+        def fd1(t):
+            r1, r2 = raw.copy(), raw.copy()
+            hh = 1e-3 * max(1.0, abs(float(raw[0, t])))
+            r1[0, t] = np.float32(raw[0, t] + hh)
+            r2[0, t] = np.float32(raw[0, t] - hh)
+            return (oracle.dm_crb(r1, t64, wc, w64) - oracle.dm_crb(r2, t64, wc, w64)) / (
+                float(r1[0, t]) - float(r2[0, t]))
+        t0 = time.time()
+        with cf.ThreadPoolExecutor(max_workers=16) as ex:
+            fds = np.array(list(ex.map(fd1, ent)))
+        crb_fd = {"entries": 32, "max_rel_err": float(np.max(np.abs(gfa[ent] - fds)) /
+                                                       float(np.max(np.abs(fds))))}
+        log(f"CRB flip-angle FD check: max rel err {crb_fd['max_rel_err']:.2e} "
+            f"({time.time() - t0:.1f} s)")
     jf = float(V) * T * w * FLOP_JAC
     af = float(V) * T * w * FLOP_ADJ
     res = {
@@ -344,13 +399,96 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
                          "cost_monotone": bool(np.all(np.diff(cost, axis=1) <= 0)),
                          "median_cost_ratio": float(np.median(cost[:, -1] / cost[:, 0]))},
         "crb": {"ms": crb_ms, "tissues": nt, "w": wc,
-                "loss": float(d_loss.sum().item())},
+                "loss": float(d_loss.sum().item()), "fd_check": crb_fd},
+        "jacobian_fd_check": fd_err,
     }
     log(f"differentiation: jacobian {jac_ms:.1f} ms, adjoint {adj_ms:.1f} ms, "
         f"GN x{iters} {gn_ms:.1f} ms, CRB {crb_ms:.1f} ms")
     del d_truth, g_th, g_seq
     torch.cuda.empty_cache()
     return res
+
+
+GRID_C1 = ("100:20:2000", "20:10:300", "0", "1")
+FLOP_PER_STATE_FISP = 78
+
+
+def config_fisp(sm, torch, np, synth, T, stream, timed, cpu):
+    """The CPU-runnable configuration: EPG-FISP, T1 100:20:2000, T2 20:10:300,
+    w = 20, T = 1000; 64 x 64 voxels (PD ~ U(0.5, 1), SNR 30 on the atom RMS)
+    matched in FP32.  Atoms are normalised (explicit flag; the reference
+    normalises only SSFP/ROA) so that the argmax recovers the truth."""
+    w = 20
+    raw = synth.csv_columns(synth.sequence(T, seed=0), "epg_fisp")
+    params, A, natoms = sm.build_params(*GRID_C1)
+    d_params = torch.from_numpy(params).cuda()
+    d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
+    run = lambda: sm.dict_ex("epg_fisp", raw, T, d_params, A, natoms, w, d_atoms, normalize=1,
+                             stream=stream)
+    run()
+    gen_ms = timed(run, 5)
+    rng = np.random.default_rng(30)
+    V = 64 * 64
+    truth = rng.integers(0, A, V)
+    pd = rng.uniform(0.5, 1.0, V).astype(np.float32)
+    clean = d_atoms[torch.from_numpy(truth).cuda()] * torch.from_numpy(pd).cuda()[:, None]
+    rms = clean.abs().pow(2).mean(dim=1).sqrt()
+    g = torch.Generator(device="cuda").manual_seed(31)
+    noise = torch.complex(torch.randn(V, T, device="cuda", generator=g),
+                          torch.randn(V, T, device="cuda", generator=g))
+    d_sig = (clean + noise * (rms / (30.0 * 2 ** 0.5))[:, None]).to(torch.complex64).contiguous()
+    d_maps = torch.empty((V, 5), device="cuda")
+    d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+    mrun = lambda: sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T, A, natoms, d_sig, V,
+                               d_maps, d_idx=d_idx, stream=stream)
+    mrun()
+    m_ms = timed(mrun, 5)
+    idx = d_idx.cpu().numpy()
+    res = {"workload": f"epg_fisp {A} atoms (T1 {GRID_C1[0]}, T2 {GRID_C1[1]}) x T={T} x w={w}, "
+                       f"FP32 matching of 64x64 voxels", "atoms": A, "voxels": V,
+           "gen_ms": gen_ms, "gen_atoms_per_s": A / (gen_ms * 1e-3),
+           "gen_tflops_algorithmic": A * T * w * FLOP_PER_STATE_FISP / (gen_ms * 1e-3) / 1e12,
+           "match_ms": m_ms, "match_voxels_per_s": V / (m_ms * 1e-3),
+           "match_truth_agree": float(np.mean(idx == truth))}
+    if cpu:
+        import oracle
+        seq = oracle.seq_prepare("epg_fisp", raw)
+        t0 = oracle.wall()
+        atoms = oracle.epg("epg_fisp", seq, params, natoms, w)
+        dt_gen = oracle.wall() - t0
+        a32 = (atoms / np.linalg.norm(atoms, axis=1, keepdims=True)).astype(np.complex64)
+        sig = d_sig.cpu().numpy()
+        t0 = oracle.wall()
+        ridx, _, _ = oracle.match_slice(a32, sig)
+        dt_m = oracle.wall() - t0
+        res["cpu"] = {"gen_atoms_per_s": A / dt_gen, "match_voxels_per_s": V / dt_m,
+                      "cores": oracle.num_threads(), "kind": "port",
+                      "sample": "full configuration (all atoms, all voxels), FP64 oracle",
+                      "argmax_equal_gpu": float(np.mean(ridx == idx))}
+    log(f"FISP config: gen {gen_ms:.3f} ms, match {m_ms:.3f} ms")
+    return res
+
+
+def roa_iso(sm, torch, np, synth, T, stream, timed):
+    """ROA Bloch variant with 100 isochromats on a 50,000-atom seeded subset
+    of the SSFP grid, T = 1000."""
+    raw = synth.csv_columns(synth.sequence(T, seed=0), "roa_ssfp")
+    params, A, natoms = sm.build_params(*GRID)
+    rng = np.random.default_rng(40)
+    sub = params[np.sort(rng.choice(natoms, 50000, replace=False))]
+    n = len(sub)
+    d_params = torch.from_numpy(np.ascontiguousarray(sub)).cuda()
+    d_atoms = torch.empty((n, T), dtype=torch.complex64, device="cuda")
+    run = lambda: sm.dict_ex("roa_ssfp", raw, T, d_params, n, n, 30, d_atoms, n_iso=100,
+                             iso_spread_hz=100.0, stream=stream)
+    run()
+    ms = timed(run, 2)
+    flops = float(n) * 100 * T * 31
+    log(f"ROA 100 isochromats: {ms:.2f} ms")
+    return {"workload": f"roa_ssfp {n} atoms x 100 isochromats (100 Hz band) x T={T}",
+            "ms": ms, "atoms_per_s": n / (ms * 1e-3),
+            "tflops_algorithmic": flops / (ms * 1e-3) / 1e12,
+            "frac_fp32_peak": flops / (ms * 1e-3) / 1e12 / PEAK_FP32, "flop_per_iso_tr": 31}
 
 
 def measured_traffic(key: str):
@@ -375,7 +513,7 @@ def main() -> int:
     ap.add_argument("--skip-match", action="store_true")
     ap.add_argument("--skip-lowp", action="store_true", help="skip the BF16 precision form")
     ap.add_argument("--skip-diff", action="store_true", help="skip the differentiation form")
-    ap.add_argument("--oracle-voxels", type=int, default=256,
+    ap.add_argument("--oracle-voxels", type=int, default=4096,
                     help="voxels of the precision form checked against the FP64 oracle")
     ap.add_argument("--profile", action="store_true",
                     help="one EPG launch, one FP32 matching call, the precision-form matchers "
@@ -486,7 +624,10 @@ def main() -> int:
                                  "algorithmic_flops_per_launch": mflops}
     if not args.skip_diff:
         out["differentiation"] = differentiation(sm, torch, np, raw, T, w, d_atoms, d_params,
-                                                 params, A, natoms, stream, timed)
+                                                 params, A, natoms, stream, timed,
+                                                 0 if args.no_cpu else 1024)
+    out["fisp_config"] = config_fisp(sm, torch, np, synth, T, stream, timed, not args.no_cpu)
+    out["roa_isochromats"] = roa_iso(sm, torch, np, synth, T, stream, timed)
     if not args.skip_lowp:
         if not args.skip_match:
             del d_sig, d_b1, d_maps, d_idx
