@@ -584,44 +584,56 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                 if (SSPLIT) *reinterpret_cast<u32x4 *>(&lXl[o]) = gxl[p];
             }
         };
+        // All fragments of a chunk are read from LDS before its MFMAs, so the
+        // LDS latency is paid once per chunk rather than once per A block.
         auto compute = [&](int b) {
+            constexpr int KS = KC / 16;
+            u32x4 fx[KS][4], fxl[KS][4], fa[KS][8], fal[KS][8];
 #pragma unroll
-            for (int ks = 0; ks < KC / 16; ks++) {
+            for (int ks = 0; ks < KS; ks++) {
                 const int ko = ks * 16 + g * 4;
-                u32x4 fx[4], fy[4], fxl[4], fyl[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int o = b * BX + (w * 64 + v * 16 + r16) * LSB + ko;
-                    fx[v] = *reinterpret_cast<const u32x4 *>(&lX[o]);
-                    fy[v] = swap_neg4(fx[v]);
-                    if (SSPLIT) {
-                        fxl[v] = *reinterpret_cast<const u32x4 *>(&lXl[o]);
-                        fyl[v] = swap_neg4(fxl[v]);
-                    }
+                    fx[ks][v] = *reinterpret_cast<const u32x4 *>(&lX[o]);
+                    if (SSPLIT) fxl[ks][v] = *reinterpret_cast<const u32x4 *>(&lXl[o]);
                 }
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const int o = b * BA + (i * 16 + r16) * LSB + ko;
-                    const u32x4 fa = *reinterpret_cast<const u32x4 *>(&lA[o]);
-                    u32x4 fal;
-                    if (ASPLIT) fal = *reinterpret_cast<const u32x4 *>(&lAl[o]);
+                    fa[ks][i] = *reinterpret_cast<const u32x4 *>(&lA[o]);
+                    if (ASPLIT) fal[ks][i] = *reinterpret_cast<const u32x4 *>(&lAl[o]);
+                }
+            }
+            // keep the scheduler from sinking the reads next to their MFMAs
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                u32x4 fy[4], fyl[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    fy[v] = swap_neg4(fx[ks][v]);
+                    if (SSPLIT) fyl[v] = swap_neg4(fxl[ks][v]);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        acc_re[i][v] = mfma_bf16(fa, fx[v], acc_re[i][v]);
-                        acc_im[i][v] = mfma_bf16(fa, fy[v], acc_im[i][v]);
+                        acc_re[i][v] = mfma_bf16(fa[ks][i], fx[ks][v], acc_re[i][v]);
+                        acc_im[i][v] = mfma_bf16(fa[ks][i], fy[v], acc_im[i][v]);
                     }
                     if (ASPLIT) {
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            acc_re[i][v] = mfma_bf16(fal, fx[v], acc_re[i][v]);
-                            acc_im[i][v] = mfma_bf16(fal, fy[v], acc_im[i][v]);
+                            acc_re[i][v] = mfma_bf16(fal[ks][i], fx[ks][v], acc_re[i][v]);
+                            acc_im[i][v] = mfma_bf16(fal[ks][i], fy[v], acc_im[i][v]);
                         }
                     }
                     if (SSPLIT) {
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            acc_re[i][v] = mfma_bf16(fa, fxl[v], acc_re[i][v]);
-                            acc_im[i][v] = mfma_bf16(fa, fyl[v], acc_im[i][v]);
+                            acc_re[i][v] = mfma_bf16(fa[ks][i], fxl[ks][v], acc_re[i][v]);
+                            acc_im[i][v] = mfma_bf16(fa[ks][i], fyl[v], acc_im[i][v]);
                         }
                     }
                 }
