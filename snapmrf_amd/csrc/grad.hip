@@ -10,9 +10,10 @@
 // of its inputs; the flip angles of an atom are alpha_n * B1, the CSV unit
 // transform (functions.cu:1242-1261) is folded into the per-TR table.
 //
-// Layout: one wave per atom, lane k holds EPG state k (F+_k, F-_k, Z_k) and
-// its ND tangents (w <= 64), so an atom's whole state sits in 6 (1 + ND)
-// registers per lane; the dephasing shift is a cross-lane move.  Per TR n:
+// Layout: one group of SUB lanes per atom (SUB = 32 for w <= 32, two atoms
+// per wave; else 64), lane k holds EPG state k (F+_k, F-_k, Z_k) and its ND
+// tangents, so an atom's whole state sits in 6 (1 + ND) registers per lane;
+// the dephasing shift is a cross-lane move inside the group.  Per TR n:
 //   RF(alpha_n B1, phi_n) -> relax(TE_n) -> phase(TE_n) -> s_n = F+_0 (lane 0)
 //   -> phase(TR_n - TE_n) -> dephase -> relax(TR_n - TE_n)
 // Arithmetic is FP32; FP64 is used only for the 4x4 Gauss-Newton solve and
@@ -45,20 +46,34 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) { return {a.x * s, 
 __device__ __forceinline__ float2 c_i(float2 a) { return {-a.y, a.x}; }      // i * a
 __device__ __forceinline__ float re_dot(float2 l, float2 v) { return l.x * v.x + l.y * v.y; }  // Re(conj(l) v)
 
+// Lane groups of SUB lanes (32 or 64) each hold one atom; shifts,
+// broadcasts and sums stay inside the group.
+template <int SUB>
 __device__ __forceinline__ float2 shfl_up1(float2 v)
 {
-    return {__shfl_up(v.x, 1), __shfl_up(v.y, 1)};
+    return {__shfl_up(v.x, 1, SUB), __shfl_up(v.y, 1, SUB)};
 }
+template <int SUB>
 __device__ __forceinline__ float2 shfl_down1(float2 v)
 {
-    return {__shfl_down(v.x, 1), __shfl_down(v.y, 1)};
+    return {__shfl_down(v.x, 1, SUB), __shfl_down(v.y, 1, SUB)};
 }
-__device__ __forceinline__ float2 bcast0(float2 v) { return {__shfl(v.x, 0), __shfl(v.y, 0)}; }
+template <int SUB>
+__device__ __forceinline__ float2 bcast0(float2 v)
+{
+    return {__shfl(v.x, 0, SUB), __shfl(v.y, 0, SUB)};
+}
+template <int SUB>
+__device__ __forceinline__ float bcast0(float v)
+{
+    return __shfl(v, 0, SUB);
+}
 
-__device__ __forceinline__ float wave_sum(float v)
+template <int SUB>
+__device__ __forceinline__ float group_sum(float v)
 {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = SUB / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, SUB);
     return v;
 }
 
@@ -210,13 +225,13 @@ __device__ __forceinline__ void f_phase(Aug<ND> &y, float2 P, float t)
 
 // dephase_gradients (functions.cu:619-668): F+_k <- F+_{k-1}, F-_k <- F-_{k+1},
 // F+_0 <- conj(F-_0), F-_{w-1} <- 0; lanes >= w stay zero.
-template <int ND>
+template <int ND, int SUB>
 __device__ __forceinline__ void f_dephase(Aug<ND> &y, int k, int w)
 {
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
-        float2 m = shfl_down1(y.fm[j]);
-        float2 pl = shfl_up1(y.fp[j]);
+        float2 m = shfl_down1<SUB>(y.fm[j]);
+        float2 pl = shfl_up1<SUB>(y.fp[j]);
         if (k >= w - 1) m = {0.f, 0.f};
         if (k == 0) pl = c_conj(m);
         if (k >= w) pl = {0.f, 0.f};
@@ -225,7 +240,7 @@ __device__ __forceinline__ void f_dephase(Aug<ND> &y, int k, int w)
     }
 }
 
-template <int ND>
+template <int ND, int SUB>
 __device__ __forceinline__ void f_tr(Aug<ND> &y, const TrK &c, const Atom &p, int k, int w,
                                      Aug<ND> *at_readout)
 {
@@ -234,7 +249,7 @@ __device__ __forceinline__ void f_tr(Aug<ND> &y, const TrK &c, const Atom &p, in
     f_phase(y, c.pa, c.te);
     if (at_readout) *at_readout = y;
     f_phase(y, c.pb, c.t2);
-    f_dephase(y, k, w);
+    f_dephase<ND, SUB>(y, k, w);
     f_relax(y, c.e1b, c.e2b, c.t2, p, k == 0);
 }
 
@@ -312,14 +327,14 @@ __device__ __forceinline__ void r_relax(Aug<ND> &l, const Aug<ND> &x, float e1, 
     l.z[0] = lz0;
 }
 
-template <int ND>
+template <int ND, int SUB>
 __device__ __forceinline__ void r_dephase(Aug<ND> &l, int k, int w)
 {
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
-        const float2 lp0 = bcast0(l.fp[j]);
-        float2 p = shfl_down1(l.fp[j]);
-        float2 m = shfl_up1(l.fm[j]);
+        const float2 lp0 = bcast0<SUB>(l.fp[j]);
+        float2 p = shfl_down1<SUB>(l.fp[j]);
+        float2 m = shfl_up1<SUB>(l.fm[j]);
         if (k >= w - 1) p = {0.f, 0.f};
         if (k == 0 || k >= w) m = {0.f, 0.f};
         if (k == 1) m = c_add(m, c_conj(lp0));
@@ -455,20 +470,29 @@ __device__ __forceinline__ Atom load_atom(const float *theta, size_t a)
 
 // ---------------------------------------------------------------------------
 // Jacobian: s [n][T] and J [n][4][T] (either may be null)
+// atom of the calling lane's group and the lane's state index
+template <int SUB>
+__device__ __forceinline__ size_t group_atom()
+{
+    return ((size_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) * (64 / SUB) +
+           ((threadIdx.x & 63) / SUB);
+}
+
+template <int SUB>
 __global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T, int w,
                                                           const float *theta, size_t n,
                                                           float2 *sig, float2 *jac)
 {
-    const size_t a = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const size_t a = group_atom<SUB>();
     if (a >= n) return;
-    const int k = threadIdx.x & 63;
+    const int k = threadIdx.x & (SUB - 1);
     const Atom p = load_atom(theta, a);
     Aug<4> y;
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
         const TrK c = tr_coef(seq[t], p);
         Aug<4> r;
-        f_tr(y, c, p, k, w, &r);
+        f_tr<4, SUB>(y, c, p, k, w, &r);
         if (k == 0) {
             if (sig) sig[a * T + t] = r.fp[0];
             if (jac)
@@ -526,12 +550,12 @@ __device__ bool inv4(const double *A, double *X)
     return true;
 }
 
-template <int ND>
+template <int ND, int SUB>
 __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
 {
-    const size_t a = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const size_t a = group_atom<SUB>();
     if (a >= A.n) return;
-    const int k = threadIdx.x & 63;
+    const int k = threadIdx.x & (SUB - 1);
     const int w = A.w, T = A.T;
     const int C = 6 * (1 + ND) * w;
     float *ck = A.scratch + a * (size_t)(A.nck + kCkpt) * C;
@@ -549,7 +573,7 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
         const TrK c = tr_coef(A.seq[t], p);
         if constexpr (ND > 0) {
             Aug<ND> r;
-            f_tr(y, c, p, k, w, &r);
+            f_tr<ND, SUB>(y, c, p, k, w, &r);
             if (k == 0) {
                 int i = 0;
 #pragma unroll
@@ -558,7 +582,7 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
                     for (int e = d; e < 4; e++) F[i++] += re_dot(r.fp[1 + d], r.fp[1 + e]);
             }
         } else {
-            f_tr<ND>(y, c, p, k, w, nullptr);
+            f_tr<ND, SUB>(y, c, p, k, w, nullptr);
         }
     }
     // loss and dL/dF (ND = 4), on lane 0 in FP64, broadcast as G
@@ -593,7 +617,7 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
             A.loss[a] = loss;
         }
 #pragma unroll
-        for (int i = 0; i < 10; i++) G[i] = __shfl(G[i], 0);
+        for (int i = 0; i < 10; i++) G[i] = bcast0<SUB>(G[i]);
     }
 
     // reverse sweep, segment by segment
@@ -608,7 +632,7 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
         st_load(ck + (size_t)sgi * C, w, k, y);
         for (int t = t0; t < t1; t++) {
             st_store(seg + (size_t)(t - t0) * C, w, k, y);
-            f_tr<ND>(y, tr_coef(A.seq[t], p), p, k, w, nullptr);
+            f_tr<ND, SUB>(y, tr_coef(A.seq[t], p), p, k, w, nullptr);
         }
         for (int t = t1 - 1; t >= t0; t--) {
             Aug<ND> yn;
@@ -622,10 +646,10 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
                 f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
                 f_phase(x, c.pa, c.te);
                 f_phase(x, c.pb, c.t2);
-                f_dephase(x, k, w);
+                f_dephase<ND, SUB>(x, k, w);
                 r_relax(l, x, c.e1b, c.e2b, c.t2, p, k == 0, &gt, gth);
             }
-            r_dephase(l, k, w);
+            r_dephase<ND, SUB>(l, k, w);
             // phase(TR - TE) and the readout: input = phase_a(relax_a(rf(yn)))
             {
                 Aug<ND> x = yn;
@@ -669,8 +693,8 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
             }
             r_rf(l, yn, c, p.B1, &ga, gth);
             if (A.grad_seq) {
-                ga = wave_sum(ga);
-                gt = wave_sum(gt);
+                ga = group_sum<SUB>(ga);
+                gt = group_sum<SUB>(gt);
                 if (k == 0) {
                     atomicAdd(&A.grad_seq[t], ga * A.fa_unit);
                     atomicAdd(&A.grad_seq[T + t], gt);
@@ -690,7 +714,7 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
             gth[1] += re_dot(l.fp[0], c_conj(t2)) + re_dot(l.fm[0], t2);
         }
 #pragma unroll
-        for (int d = 0; d < 4; d++) gth[d] = wave_sum(gth[d]);
+        for (int d = 0; d < 4; d++) gth[d] = group_sum<SUB>(gth[d]);
         if (k == 0)
 #pragma unroll
             for (int d = 0; d < 4; d++) A.grad_theta[a * 4 + d] = gth[d];
@@ -719,7 +743,7 @@ struct GnSums {
     float2 Sx, U[4], W[4], D[10];
 };
 
-template <int ND>
+template <int ND, int SUB>
 __device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const float2 *x, int k)
 {
     GnSums g{};
@@ -727,7 +751,7 @@ __device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const flo
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
         Aug<ND> r;
-        f_tr(y, tr_coef(seq[t], p), p, k, w, &r);
+        f_tr<ND, SUB>(y, tr_coef(seq[t], p), p, k, w, &r);
         if (k == 0) {
             const float2 s = r.fp[0], xv = x[t];
             g.S += re_dot(s, s);
@@ -766,21 +790,22 @@ __device__ __forceinline__ Atom make_atom(const float *th)
     return p;
 }
 
+template <int SUB>
 __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
 {
-    const size_t v = (size_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const size_t v = group_atom<SUB>();
     if (v >= A.V) return;
-    const int k = threadIdx.x & 63;
+    const int k = threadIdx.x & (SUB - 1);
     const float2 *x = A.sig + v * A.T;
     float xx = 0.f;
-    for (int t = k; t < A.T; t += 64) xx += re_dot(x[t], x[t]);
-    xx = wave_sum(xx);
+    for (int t = k; t < A.T; t += SUB) xx += re_dot(x[t], x[t]);
+    xx = group_sum<SUB>(xx);
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
-    GnSums g = gn_pass<4>(A.seq, A.T, A.w, make_atom(th), x, k);
+    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
     float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
-    c = __shfl(c, 0);
+    c = bcast0<SUB>(c);
     if (k == 0 && A.cost) A.cost[v * (A.iters + 1)] = c;
     double lam = 1e-3;
     for (int it = 0; it < A.iters; it++) {
@@ -814,20 +839,20 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
             for (int d = 0; d < 4; d++) trial[d] = (float)(th[d] + dl[d]);
         }
 #pragma unroll
-        for (int d = 0; d < 4; d++) trial[d] = __shfl(trial[d], 0);
+        for (int d = 0; d < 4; d++) trial[d] = bcast0<SUB>(trial[d]);
         float ct = INFINITY;
         if (theta_ok(trial)) {
-            const GnSums h = gn_pass<0>(A.seq, A.T, A.w, make_atom(trial), x, k);
+            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k);
             ct = xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
-            ct = __shfl(ct, 0);
+            ct = bcast0<SUB>(ct);
         }
         if (ct < c) {
 #pragma unroll
             for (int d = 0; d < 4; d++) th[d] = trial[d];
             c = ct;
             lam = fmax(lam * 0.1, 1e-9);
-            if (it + 1 < A.iters) g = gn_pass<4>(A.seq, A.T, A.w, make_atom(th), x, k);
-            else g = gn_pass<0>(A.seq, A.T, A.w, make_atom(th), x, k);
+            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
+            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
         } else {
             lam = fmin(lam * 10.0, 1e12);
         }
@@ -842,6 +867,17 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
 
 }  // namespace
 
+namespace {
+
+// work-groups for n atoms at 64 / SUB atoms per wave
+dim3 groups(size_t n, int sub)
+{
+    const size_t per = (size_t)kWaves * (64 / sub);
+    return dim3((unsigned)((n + per - 1) / per));
+}
+
+}  // namespace
+
 size_t diff_scratch_floats(int nd, int T, int w)
 {
     const int nck = (T + kCkpt - 1) / kCkpt;
@@ -852,8 +888,10 @@ int launch_jacobian(const DTr *seq, int T, int w, const float *theta, size_t n, 
                     float2 *jac, hipStream_t s)
 {
     if (n == 0) return 0;
-    jac_kernel<<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(seq, T, w, theta,
-                                                                                  n, sig, jac);
+    if (w <= 32)
+        jac_kernel<32><<<groups(n, 32), 64 * kWaves, 0, s>>>(seq, T, w, theta, n, sig, jac);
+    else
+        jac_kernel<64><<<groups(n, 64), 64 * kWaves, 0, s>>>(seq, T, w, theta, n, sig, jac);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "jacobian launch: %s", hipGetErrorString(e));
 }
@@ -874,7 +912,8 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.grad_seq = grad_seq;
     A.fa_unit = fa_unit;
     A.scratch = scratch;
-    rev_kernel<0><<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    if (w <= 32) rev_kernel<0, 32><<<groups(n, 32), 64 * kWaves, 0, s>>>(A);
+    else rev_kernel<0, 64><<<groups(n, 64), 64 * kWaves, 0, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "adjoint launch: %s", hipGetErrorString(e));
 }
@@ -895,7 +934,8 @@ int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const
     for (int d = 0; d < 4; d++) A.wts[d] = wts[d];
     A.loss = loss;
     A.scratch = scratch;
-    rev_kernel<4><<<dim3((unsigned)((n + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    if (w <= 32) rev_kernel<4, 32><<<groups(n, 32), 64 * kWaves, 0, s>>>(A);
+    else rev_kernel<4, 64><<<groups(n, 64), 64 * kWaves, 0, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "crb launch: %s", hipGetErrorString(e));
 }
@@ -914,7 +954,8 @@ int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *
     A.theta = theta;
     A.pd = pd;
     A.cost = cost;
-    gn_kernel<<<dim3((unsigned)((V + kWaves - 1) / kWaves)), 64 * kWaves, 0, s>>>(A);
+    if (w <= 32) gn_kernel<32><<<groups(V, 32), 64 * kWaves, 0, s>>>(A);
+    else gn_kernel<64><<<groups(V, 64), 64 * kWaves, 0, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "gn launch: %s", hipGetErrorString(e));
 }
