@@ -36,7 +36,7 @@ constexpr int AT = 128;          // atoms per sub-tile
 constexpr int KC = 16;           // timepoints per K chunk
 constexpr int KF = 2 * KC;       // floats per row per chunk
 constexpr int LS = KF + 4;       // LDS row stride (floats), padded for ds_read_b128
-constexpr int kAtomChunk = 8192; // atoms per work-group (grid.y splits the slice)
+constexpr int kAtomChunk = 8192; // max atoms per work-group (grid.y splits the slice)
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -55,6 +55,7 @@ struct MatchArgs {
     const int *ntiles;
     unsigned long long *keys;   // [V]
     size_t natoms;              // atoms per slice
+    size_t chunk;               // atoms per work-group (multiple of AT)
     int T;
 };
 
@@ -87,9 +88,9 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
     const int tile_id = blockIdx.x;
     if (tile_id >= *a.ntiles) return;
     const Tile tile = a.tiles[tile_id];
-    const size_t a_begin = (size_t)blockIdx.y * kAtomChunk;
+    const size_t a_begin = (size_t)blockIdx.y * a.chunk;
     if (a_begin >= a.natoms) return;
-    const size_t a_end = (a_begin + kAtomChunk < a.natoms) ? a_begin + kAtomChunk : a.natoms;
+    const size_t a_end = (a_begin + a.chunk < a.natoms) ? a_begin + a.chunk : a.natoms;
     const size_t slice_base = (size_t)tile.slice * a.natoms;
 
     __shared__ __attribute__((aligned(16))) float lds_a[AT * LS];
@@ -984,7 +985,15 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     a.keys = sc.keys;
     a.natoms = nat;
     a.T = l.T;
-    const size_t chunks = (nat + kAtomChunk - 1) / kAtomChunk;
+    // at most kAtomChunk atoms per work-group; small problems split the
+    // atoms further so that there are about 2048 work-groups
+    size_t want = (2048 + sc.max_tiles - 1) / sc.max_tiles;
+    size_t chunk = (nat + want - 1) / want;
+    chunk = (chunk + AT - 1) / AT * AT;
+    if (chunk > (size_t)kAtomChunk) chunk = kAtomChunk;
+    a.chunk = chunk;
+    const size_t chunks = (nat + chunk - 1) / chunk;
+    if (chunks > 65535) return mrf_fail(MRF_E_ARG, "dictionary slice too large");
     dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
     match_fp32_kernel<<<grid, kBlock, 0, s>>>(a);
     maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);

@@ -44,7 +44,7 @@ void print_usage()
     std::fprintf(stderr, "\t-T, --timepoints n\t\t number of time points\n");
     std::fprintf(stderr, "\t-G, --gridsize n\t\t set GPU gridsize (accepted, unused)\n");
     std::fprintf(stderr, "\t-B, --blocksize n\t\t set GPU blocksize (accepted, unused)\n");
-    std::fprintf(stderr, "\t    --precision fp32|bf16|split3\t matcher precision (default fp32)\n");
+    std::fprintf(stderr, "\t    --precision fp32|bf16|bf16_top16|split2|split3\t matcher precision (default fp32)\n");
     std::fprintf(stderr, "\t-h\t\t\t\t show this help\n");
 }
 
@@ -143,6 +143,8 @@ int main(int argc, char *argv[])
     }
     int mode = MRF_MATCH_FP32;
     if (!std::strcmp(precision, "bf16")) mode = MRF_MATCH_BF16_TOP8;
+    else if (!std::strcmp(precision, "bf16_top16")) mode = MRF_MATCH_BF16_TOP16;
+    else if (!std::strcmp(precision, "split2")) mode = MRF_MATCH_SPLIT2;
     else if (!std::strcmp(precision, "split3")) mode = MRF_MATCH_SPLIT3;
     else if (std::strcmp(precision, "fp32")) {
         std::fprintf(stderr, "mrf: unknown precision '%s'\n", precision);
@@ -251,9 +253,12 @@ int main(int argc, char *argv[])
             void *hi = nullptr, *lo = nullptr;
             mrf_complex *d_sig = nullptr;
             float *d_b1 = nullptr;
-            hipMalloc(&hi, NATOMS * nreps * 4);
-            if (mode == MRF_MATCH_SPLIT3) hipMalloc(&lo, NATOMS * nreps * 4);
-            hipMalloc((void **)&d_sig, nvoxels * nt * sizeof(mrf_complex));
+            const size_t lowp_bytes = NATOMS * mrf_lowp_ld(nreps) * 4;
+            const bool need_lo = mode == MRF_MATCH_SPLIT3 || mode == MRF_MATCH_SPLIT2;
+            if (hipMalloc(&hi, lowp_bytes) != hipSuccess ||
+                (need_lo && hipMalloc(&lo, lowp_bytes) != hipSuccess) ||
+                hipMalloc((void **)&d_sig, nvoxels * nt * sizeof(mrf_complex)) != hipSuccess)
+                return fail(EX_SOFTWARE, "device allocation");
             hipMemcpy(d_sig, img.data, nvoxels * nt * sizeof(mrf_complex), hipMemcpyHostToDevice);
             if (in_b1map) {
                 mrf_ra_t rb;
