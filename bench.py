@@ -233,8 +233,18 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
             ridx, _ = fp64_match(samp)
             lowp_right = float(np.mean(out[name][1][samp] == ridx))
             fp32_right = float(np.mean(i32[samp] == ridx))
+            # FP64 score margin by which the FP32 pick loses where it is wrong:
+            # (|<a_best, s>|^2 - |<a_fp32, s>|^2) / |<a_best, s>|^2
+            sig = d_sig[torch.from_numpy(samp).cuda()].cpu().numpy().astype(np.complex128)
+            wrong = i32[samp] != ridx
+            def score(ix):
+                return np.abs(np.sum(np.conj(atoms[ix].astype(np.complex128)) * sig[wrong], axis=1)) ** 2
+            gap = ((score(ridx[wrong]) - score(i32[samp][wrong])) / score(ridx[wrong])
+                   if np.any(wrong) else np.zeros(1))
             arb[name] = {"disagree": len(diff), "sampled": len(samp),
                          "fp64_picks_lowp": lowp_right, "fp64_picks_fp32": fp32_right,
+                         "fp32_wrong_fp64_margin_max": float(np.max(gap)),
+                         "fp32_wrong_fp64_margin_median": float(np.median(gap)),
                          "agree_fp32_or_fp64": 1.0 - len(diff) / V * (1.0 - lowp_right)}
             res[name]["agree_fp32_or_fp64"] = arb[name]["agree_fp32_or_fp64"]
             log(f"FP64 arbitration {name}: {len(diff)} disagreements, lowp right "

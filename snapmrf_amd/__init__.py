@@ -206,8 +206,9 @@ def ra_read(path: str):
         dt = {(3, 4): np.float32, (3, 8): np.float64, (4, 8): np.complex64,
               (4, 16): np.complex128, (1, 4): np.int32, (1, 8): np.int64,
               (2, 4): np.uint32, (2, 8): np.uint64}.get((ra.eltype, ra.elbyte), np.uint8)
-        raw = ctypes.string_at(ra.data, ra.size)
-        arr = np.frombuffer(raw, dtype=dt).copy()
+        raw = (np.ctypeslib.as_array(ra.data, shape=(ra.size,)) if ra.size
+               else np.zeros(0, np.uint8))
+        arr = raw.view(dt).copy()
         if dt is not np.uint8:
             arr = arr.reshape(dims[::-1])
         hdr = dict(flags=ra.flags, eltype=ra.eltype, elbyte=ra.elbyte, size=ra.size, dims=dims)

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace mrf {
 namespace {
@@ -587,9 +588,9 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         };
         // All fragments of a chunk are read from LDS before its MFMAs, so the
         // LDS latency is paid once per chunk rather than once per A block.
-        auto compute = [&](int b) {
-            constexpr int KS = KC / 16;
-            u32x4 fx[KS][4], fxl[KS][4], fa[KS][8], fal[KS][8];
+        constexpr int KS = KC / 16;
+        u32x4 fx[KS][4], fxl[KS][4], fa[KS][8], fal[KS][8];
+        auto read_frags = [&](int b) {
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
                 const int ko = ks * 16 + g * 4;
@@ -606,8 +607,8 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                     if (ASPLIT) fal[ks][i] = *reinterpret_cast<const u32x4 *>(&lAl[o]);
                 }
             }
-            // keep the scheduler from sinking the reads next to their MFMAs
-            __builtin_amdgcn_sched_barrier(0);
+        };
+        auto mfmas = [&]() {
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
                 u32x4 fy[4], fyl[4];
@@ -640,15 +641,35 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                 }
             }
         };
+        // One K chunk: its fragments are read, then its MFMAs run while the
+        // next chunk is staged into the other LDS buffer.  The global loads
+        // are spread over the first half of the MFMAs and the LDS stores
+        // over the second half, so the staging never holds up MFMA issue.
+        constexpr int NMF = KS * 8 * 4 * 2 * NPROD;
+        constexpr int NLD = PA * (ASPLIT ? 2 : 1) + PX * (SSPLIT ? 2 : 1);
+        constexpr int MPL = NMF / 2 / NLD;
+        // The last chunk restages itself (in bounds, unused) so that every
+        // step has the same straight-line body.
         const int nk = a.nk;
-        // chunk c + 1 is fetched while chunk c is multiplied
         stage_load(0);
         stage_store(0);
         __syncthreads();
         for (int c = 0; c < nk; c++) {
-            if (c + 1 < nk) stage_load(c + 1);
-            compute(c & 1);
-            if (c + 1 < nk) stage_store((c + 1) & 1);
+            read_frags(c & 1);
+            __builtin_amdgcn_sched_barrier(0);
+            stage_load(c + 1 < nk ? c + 1 : c);
+            mfmas();
+            stage_store((c + 1) & 1);
+#pragma unroll
+            for (int j = 0; j < NLD; j++) {
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // VMEM read
+                __builtin_amdgcn_sched_group_barrier(0x008, MPL, 0); // MFMA
+            }
+#pragma unroll
+            for (int j = 0; j < NLD; j++) {
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+                __builtin_amdgcn_sched_group_barrier(0x008, MPL, 0); // MFMA
+            }
             __syncthreads();
         }
         // epilogue: fold |S|^2 of the sub-tile into this lane's top-4 lists.

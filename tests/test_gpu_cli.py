@@ -90,3 +90,45 @@ def test_cli_low_precision(gpu, tmp_path, precision):
     m32, m = res32["maps"][0], res["maps"][0]
     same = np.all(m32[:, :4] == m[:, :4], axis=1)
     assert np.mean(same) >= 0.999
+
+
+def test_cli_natoms_mode_without_b1_map(gpu, tmp_path):
+    """-N grid (src/mrf.cu:179-203: log-spaced T1/T2, symmetric-log B0, 11 B1
+    slices) and matching without a B1 map, where the slice with the largest
+    PD wins (src/functions.cu:1386-1416), i.e. the first-index argmax over all
+    atoms."""
+    Tn, Wn, N = 60, 10, 3000
+    raw = synth.csv_columns(synth.sequence(Tn, seed=6), "epg_ssfp")
+    csv = tmp_path / "seq.csv"
+    synth.write_csv(str(csv), raw)
+    params, A, natoms = sm.params_from_axes(*sm.grid_natoms_mode(N))
+    assert A // natoms == 11
+    d_params = gpu.from_numpy(params).cuda()
+    d_atoms = gpu.empty((A, Tn), dtype=gpu.complex64, device="cuda")
+    sm.MRF_dict(d_atoms, d_params, raw, Tn, Tn, A, natoms, 4, Wn, "epg_ssfp")
+    atoms = d_atoms.cpu().numpy()
+    rng = np.random.default_rng(8)
+    nx, ny = 12, 10
+    V = nx * ny
+    truth = rng.integers(0, A, V)
+    sig = atoms[truth] * rng.uniform(0.4, 1.0, V).astype(np.float32)[:, None]
+    sig = (sig + (rng.normal(size=sig.shape) + 1j * rng.normal(size=sig.shape)) * 0.002).astype(np.complex64)
+    img = tmp_path / "img.ra"
+    sm.ra_write(str(img), sig.reshape(ny, nx, Tn))
+    out = {k: str(tmp_path / f"{k}.ra") for k in ("atoms", "maps", "params")}
+    r = subprocess.run([sm.CLI_PATH, "-N", str(N), "-w", str(Wn), "-T", str(Tn),
+                        "-a", out["atoms"], "-m", out["maps"], "-p", out["params"],
+                        str(csv), str(img)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"NATOMS: {A}" in r.stdout
+    assert np.array_equal(sm.ra_read(out["params"])[0], params)
+    assert np.array_equal(sm.ra_read(out["atoms"])[0], atoms)
+    m_cli = sm.ra_read(out["maps"])[0]
+    d_maps = gpu.empty((V, 5), device="cuda")
+    sm.MRF_match(d_maps, sig, d_atoms, d_params, Tn, Tn, A, natoms, V, 4, 5, None)
+    assert np.array_equal(m_cli, d_maps.cpu().numpy())
+    # FP64 oracle: global argmax, on voxels whose top-2 gap is clear
+    ridx, rpd, rsec = oracle.match_slice(atoms, sig)
+    clear = (rpd - rsec) / rpd > 1e-5
+    assert np.all(m_cli[clear, :4] == params[ridx[clear]])
+    assert np.allclose(m_cli[clear, 4], rpd[clear], rtol=1e-4)

@@ -135,6 +135,23 @@ def test_ra_roundtrip_is_byte_identical(golden, tmp_path, name):
     assert open(out, "rb").read() == open(src, "rb").read()
 
 
+def test_ra_file_larger_than_2gib(tmp_path):
+    """A (T, A) complex64 atoms file past 2 GiB: every byte written (the
+    reference's writer over-reads there, src/ra.cu:83-88) and read back."""
+    T, A = 1000, 270_000                      # 2.16e9 bytes of data
+    arr = np.empty((A, T), dtype=np.complex64)
+    arr.view(np.uint32).ravel()[:] = np.arange(2 * A * T, dtype=np.uint32)
+    path = str(tmp_path / "big.ra")
+    sm.ra_write(path, arr)
+    assert os.path.getsize(path) == 6 * 8 + 2 * 8 + arr.nbytes
+    disk = np.memmap(path, dtype=np.uint32, mode="r", offset=64)
+    assert np.array_equal(disk[-1000:], arr.view(np.uint32).ravel()[-1000:])
+    del disk
+    back, hdr = sm.ra_read(path)
+    assert hdr["dims"] == [T, A] and hdr["size"] == arr.nbytes
+    assert np.array_equal(back.view(np.uint32), arr.view(np.uint32))
+
+
 def test_ra_rejects_bad_files(tmp_path):
     p = tmp_path / "bad.ra"
     p.write_bytes(b"\0" * 64)
