@@ -479,9 +479,10 @@ def config_fisp(sm, torch, np, synth, T, stream, timed, cpu):
     return res
 
 
-def roa_iso(sm, torch, np, synth, T, stream, timed):
+def roa_iso(sm, torch, np, synth, T, stream, timed, check):
     """ROA Bloch variant with 100 isochromats on a 50,000-atom seeded subset
-    of the SSFP grid, T = 1000."""
+    of the SSFP grid, T = 1000; with `check`, 64 of the atoms against the
+    FP64 oracle."""
     raw = synth.csv_columns(synth.sequence(T, seed=0), "roa_ssfp")
     params, A, natoms = sm.build_params(*GRID)
     rng = np.random.default_rng(40)
@@ -495,10 +496,20 @@ def roa_iso(sm, torch, np, synth, T, stream, timed):
     ms = timed(run, 2)
     flops = float(n) * 100 * T * 31
     log(f"ROA 100 isochromats: {ms:.2f} ms")
-    return {"workload": f"roa_ssfp {n} atoms x 100 isochromats (100 Hz band) x T={T}",
-            "ms": ms, "atoms_per_s": n / (ms * 1e-3),
-            "tflops_algorithmic": flops / (ms * 1e-3) / 1e12,
-            "frac_fp32_peak": flops / (ms * 1e-3) / 1e12 / PEAK_FP32, "flop_per_iso_tr": 31}
+    res = {"workload": f"roa_ssfp {n} atoms x 100 isochromats (100 Hz band) x T={T}",
+           "ms": ms, "atoms_per_s": n / (ms * 1e-3),
+           "tflops_algorithmic": flops / (ms * 1e-3) / 1e12,
+           "frac_fp32_peak": flops / (ms * 1e-3) / 1e12 / PEAK_FP32, "flop_per_iso_tr": 31}
+    if check:
+        import oracle
+        pick = np.sort(rng.choice(n, 64, replace=False))
+        ref = oracle.roa(oracle.seq_prepare("roa_ssfp", raw), sub[pick], 64, n_iso=100,
+                         spread_hz=100.0)
+        got = d_atoms[torch.from_numpy(pick).cuda()].cpu().numpy()
+        res["oracle_max_rel_err_64_atoms"] = float(
+            np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        log(f"ROA 100 isochromats vs FP64 oracle (64 atoms): {res['oracle_max_rel_err_64_atoms']:.2e}")
+    return res
 
 
 def measured_traffic(key: str):
@@ -637,7 +648,7 @@ def main() -> int:
                                                  params, A, natoms, stream, timed,
                                                  0 if args.no_cpu else 1024)
     out["fisp_config"] = config_fisp(sm, torch, np, synth, T, stream, timed, not args.no_cpu)
-    out["roa_isochromats"] = roa_iso(sm, torch, np, synth, T, stream, timed)
+    out["roa_isochromats"] = roa_iso(sm, torch, np, synth, T, stream, timed, not args.no_cpu)
     if not args.skip_lowp:
         if not args.skip_match:
             del d_sig, d_b1, d_maps, d_idx

@@ -114,7 +114,10 @@ int mrf_dict(mrf_complex *d_atoms, const float *d_params, const float *h_mrf,
 typedef struct {
     int normalize;        /* -1: reference default for the echo type, 0/1 force */
     int n_iso;            /* roa_ssfp isochromats per atom (<=1: reference) */
-    float iso_spread_hz;  /* roa_ssfp off-resonance band of the isochromats */
+    float iso_spread_hz;  /* roa_ssfp off-resonance band of the isochromats:
+                             isochromat q of n precesses at
+                             B0 + spread*((q + 1/2)/n - 1/2) Hz; the signal
+                             is their mean */
     int reserved[8];
 } mrf_dict_opts;
 

@@ -268,7 +268,9 @@ int orc_roa(const float *seq, size_t T, const float *params, size_t natoms,
             /* isochromat offset (new semantics; q-th midpoint of the band) */
             double off = (n_iso == 1) ? 0.0
                 : iso_spread_hz * (((double)q + 0.5) / (double)n_iso - 0.5);
-            float B0 = (float)((double)B0a + off);
+            /* the isochromat frequency in double (a float frequency would
+             * turn its rounding into a phase drift over the TRs) */
+            double B0 = (double)B0a + off;
             /* init_rf_pulse(180, 0, TI=0), functions.cu:1022-1047 */
             float a = (float)(180.0 * M_PI / 180.0f);
             double sa = sin((double)a), ca = cos((double)a);
@@ -289,7 +291,7 @@ int orc_roa(const float *seq, size_t T, const float *params, size_t natoms,
                 double e2 = exp(-(double)TR[i] / T2);
                 mx = e2 * nx; my = e2 * ny; mz = e1 * nz + (1.0 - e1);
                 /* fill_offresoance_operator (:1107-1127): beta is a float */
-                float beta = (float)(M_PI * (double)B0 * (double)TR[i] / 1000.0f);
+                float beta = (float)(M_PI * B0 * (double)TR[i] / 1000.0f);
                 double cb = cos((double)beta), sb = sin((double)beta);
                 double x = cb * mx + sb * my, y = cb * my - sb * mx;
                 mx = x; my = y;

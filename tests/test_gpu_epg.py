@@ -109,7 +109,9 @@ def test_roa_isochromats_vs_oracle(gpu):
     raw = synth.csv_columns(s, "roa_ssfp")
     params, A, natoms = sm.build_params("200:300:2000", "30:50:300", "-30:15:30", "1")
     seq = oracle.seq_prepare("roa_ssfp", raw)
-    for n_iso, spread in ((1, 0.0), (100, 60.0), (13, 25.0)):
+    # 13: a partial lane group; 230: more isochromats than one pass holds;
+    # T = 500 ends on a partial group of TRs
+    for n_iso, spread in ((1, 0.0), (100, 60.0), (13, 25.0), (230, 80.0)):
         got = gen(gpu, "roa_ssfp", raw, params, natoms, 2, n_iso=n_iso, iso_spread_hz=spread)
         ref = oracle.roa(seq, params, natoms, n_iso=n_iso, spread_hz=spread)
         assert rel_err(got, ref) < TOL, (n_iso, spread)
