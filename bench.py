@@ -643,6 +643,27 @@ def main() -> int:
                                  "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
                                  "algorithmic_flops_per_launch": mflops}
+        # the same generation + matching streamed: the dictionary exists one
+        # block (at most 1 GiB) at a time
+        d_maps_s = torch.empty((V, 5), device="cuda")
+        d_idx_s = torch.empty(V, dtype=torch.int32, device="cuda")
+
+        def streamed():
+            sm.dict_match_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_sig, V, d_maps_s,
+                             d_b1map=d_b1, d_idx=d_idx_s, stream=stream)
+        streamed()
+        torch.cuda.synchronize()
+        st_ms = timed(streamed, 1)
+        block = (1 << 30) // (l_b1 * T * 8) // 128 * 128
+        out["streamed_gen_match"] = {
+            "ms": st_ms, "separate_gen_plus_match_ms": gen_ms + match_ms,
+            "block_atoms_per_slice": block, "block_bytes": block * l_b1 * T * 8,
+            "dictionary_bytes": A * T * 8,
+            "idx_equal_whole_dictionary": bool(np.array_equal(d_idx_s.cpu().numpy(), idx)),
+            "maps_equal_whole_dictionary": bool(torch.equal(d_maps_s, d_maps))}
+        log(f"streamed generation + matching: {st_ms:.1f} ms "
+            f"(equal: {out['streamed_gen_match']['idx_equal_whole_dictionary']})")
+        del d_maps_s, d_idx_s
     if not args.skip_diff:
         out["differentiation"] = differentiation(sm, torch, np, raw, T, w, d_atoms, d_params,
                                                  params, A, natoms, stream, timed,

@@ -173,6 +173,22 @@ int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                  size_t natoms, const mrf_complex *d_sig, size_t V,
                  const float *d_b1map, float *d_maps, int32_t *d_idx);
 
+/* Dictionary generation streamed through the FP32 matcher: the atoms exist
+ * one block at a time (block_atoms per B1 slice, every slice; 0 = blocks of
+ * at most 1 GiB), each block is generated as mrf_dict_ex would generate it
+ * and matched at once, and the running per-voxel (score, first index) merges
+ * across blocks.  d_maps / d_idx equal mrf_dict_ex followed by mrf_match_ex
+ * (MRF_MATCH_FP32) on the same inputs, without the NATOMS x T dictionary in
+ * memory.  No counterpart in the reference, which always materialises the
+ * dictionary (src/mrf.cu:286-321).  epg_ssfp with any number of B1 slices;
+ * the other echo types need NATOMS == natoms (they fill one slice). */
+int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
+                      size_t nreps, size_t NREPS, const float *d_params,
+                      size_t NATOMS, size_t natoms, int nstates,
+                      const mrf_dict_opts *opts, const mrf_complex *d_sig, size_t V,
+                      const float *d_b1map, size_t block_atoms, float *d_maps,
+                      int32_t *d_idx);
+
 /* Builds the BF16 copies of a complex64 dictionary: d_hi = bf16(a),
  * d_lo = bf16(a - hi) (optional), round to nearest even.  Each is
  * [NATOMS][mrf_lowp_ld(T)] words of two bf16 (re in the low half, im in the

@@ -30,7 +30,7 @@ EXPORTS = (
     "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
     "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
     "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
-    "mrf_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
+    "mrf_match_ex", "mrf_dict_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
     "mrf_gn_refine", "mrf_crb_loss_grad", "mrf_device_count",
 )
 
@@ -83,6 +83,8 @@ def lib() -> ctypes.CDLL:
         "mrf_dict_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, P], I),
         "mrf_match": ([P, P, P, P, S, S, S, S, S, I, I, cs], I),
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
+        "mrf_dict_match_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, S, P,
+                               S, P, P], I),
         "mrf_lowp_prepare": ([P, P, S, S, P, P], I),
         "mrf_lowp_ld": ([S], S),
         "mrf_epg_jacobian": ([P, P, S, S, P, S, I, P, P], I),
@@ -279,6 +281,25 @@ def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_s
     check(lib().mrf_match_ex(_stream_ptr(stream), mode, _ptr(d_atoms), _ptr(d_lowp),
                              _ptr(d_lowp_lo), _ptr(d_params), T, NATOMS, natoms, _ptr(d_sig), V,
                              _ptr(d_b1map), _ptr(d_maps), _ptr(d_idx)))
+
+
+def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int,
+                  nstates: int, d_sig, V: int, d_maps, d_b1map=None, d_idx=None,
+                  block_atoms: int = 0, normalize: int = -1, n_iso: int = 1,
+                  iso_spread_hz: float = 0.0, stream=None) -> None:
+    """Dictionary generation streamed through the FP32 matcher, block_atoms
+    atoms per slice at a time (0: blocks of at most 1 GiB); the maps equal
+    dict_ex + match_ex(MRF_MATCH_FP32)."""
+    import numpy as np
+    h = np.ascontiguousarray(h_mrf, dtype=np.float32)
+    opts = MrfDictOpts()
+    opts.normalize = normalize
+    opts.n_iso = n_iso
+    opts.iso_spread_hz = iso_spread_hz
+    check(lib().mrf_dict_match_ex(_stream_ptr(stream), _enc(echo_type), h.ctypes.data, nreps,
+                                  h.shape[1], _ptr(d_params), NATOMS, natoms, nstates,
+                                  ctypes.byref(opts), _ptr(d_sig), V, _ptr(d_b1map), block_atoms,
+                                  _ptr(d_maps), _ptr(d_idx)))
 
 
 def lowp_ld(T: int) -> int:

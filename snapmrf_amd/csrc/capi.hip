@@ -92,6 +92,44 @@ void prepare_sequence(int echo, const float *h_mrf, size_t nreps, size_t NREPS,
     }
 }
 
+// Atoms of `natoms` parameter rows per slice (n_slices slices of params,
+// rows natoms apart) with the per-slice coefficient table `coef`.
+int gen_atoms(int echo, const TrCoef *coef, const float *params, size_t natoms, int n_slices,
+              int T, int nstates, const mrf_dict_opts *opts, bool h_real, int normalize,
+              float2 *atoms, double *n2, hipStream_t s)
+{
+    int rc = 0;
+    if (echo == kRoaSsfp) {
+        RoaLaunch l{};
+        l.coef = coef;
+        l.params = params;
+        l.atoms = atoms;
+        l.norm2 = n2;
+        l.natoms = natoms;
+        l.T = T;
+        l.n_iso = opts ? opts->n_iso : 1;
+        l.iso_spread_hz = opts ? opts->iso_spread_hz : 0.f;
+        rc = launch_roa(l, s);
+        if (!rc && normalize) rc = launch_normalize(atoms, n2, natoms, T, s);
+    } else {
+        EpgLaunch l{};
+        l.coef = coef;
+        l.params = params;
+        l.atoms = atoms;
+        l.norm2 = n2;
+        l.natoms = natoms;
+        l.n_slices = n_slices;
+        l.T = T;
+        l.w = nstates;
+        l.echo = echo;
+        l.generic = opts && opts->reserved[0] == 1;
+        l.h_real = h_real && echo == kEpgFisp;
+        l.normalize = normalize != 0;
+        rc = launch_epg(l, s);
+    }
+    return rc;
+}
+
 int check_device()
 {
     int n = 0;
@@ -157,36 +195,8 @@ extern "C" int mrf_dict_ex(void *stream, const char *echo_type, const float *h_m
     MRF_HIP(hipMemcpyAsync(d_seq, seq.data(), 4 * nreps * sizeof(float), hipMemcpyHostToDevice, s));
     coef_kernel<<<dim3((T + 127) / 128), 128, 0, s>>>(d_seq, T, d_params, natoms, n_slices,
                                                        echo == kEpgSsfp ? 1 : 0, d_coef);
-    int rc = 0;
-    if (echo == kRoaSsfp) {
-        RoaLaunch l{};
-        l.coef = d_coef;
-        l.params = d_params;
-        l.atoms = reinterpret_cast<float2 *>(d_atoms);
-        l.norm2 = d_n2;
-        l.natoms = natoms;
-        l.T = T;
-        l.n_iso = opts ? opts->n_iso : 1;
-        l.iso_spread_hz = opts ? opts->iso_spread_hz : 0.f;
-        rc = launch_roa(l, s);
-    } else {
-        EpgLaunch l{};
-        l.coef = d_coef;
-        l.params = d_params;
-        l.atoms = reinterpret_cast<float2 *>(d_atoms);
-        l.norm2 = d_n2;
-        l.natoms = natoms;
-        l.n_slices = n_slices;
-        l.T = T;
-        l.w = nstates;
-        l.echo = echo;
-        l.generic = opts && opts->reserved[0] == 1;
-        l.h_real = h_real && echo == kEpgFisp;
-        l.normalize = normalize != 0;
-        rc = launch_epg(l, s);
-    }
-    if (!rc && normalize && echo == kRoaSsfp)
-        rc = launch_normalize(reinterpret_cast<float2 *>(d_atoms), d_n2, n_gen, T, s);
+    int rc = gen_atoms(echo, d_coef, d_params, natoms, n_slices, T, nstates, opts, h_real,
+                       normalize, reinterpret_cast<float2 *>(d_atoms), d_n2, s);
     (void)hipFreeAsync(d_seq, s);
     (void)hipFreeAsync(d_coef, s);
     if (own_n2) (void)hipFreeAsync(d_n2, s);
@@ -250,6 +260,113 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
     int rc = launch_match(l, scratch, s);
     (void)hipFreeAsync(scratch, s);
     if (d_vs) (void)hipFreeAsync(d_vs, s);
+    return rc;
+}
+
+extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
+                                 size_t nreps, size_t NREPS, const float *d_params,
+                                 size_t NATOMS, size_t natoms, int nstates,
+                                 const mrf_dict_opts *opts, const mrf_complex *d_sig, size_t V,
+                                 const float *d_b1map, size_t block_atoms, float *d_maps,
+                                 int32_t *d_idx)
+{
+    mrf_clear_error();
+    const int echo = echo_id(echo_type);
+    if (echo < 0)
+        return mrf_fail(MRF_E_ECHO, "unknown echo type '%s'", echo_type ? echo_type : "(null)");
+    if (!h_mrf || !d_params || !d_maps || (V && !d_sig))
+        return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (nreps == 0 || nreps > NREPS) return mrf_fail(MRF_E_ARG, "need 0 < nreps <= NREPS");
+    if (nreps > 0x7fffffff) return mrf_fail(MRF_E_ARG, "nreps too large");
+    if (natoms == 0 || NATOMS == 0 || NATOMS % natoms)
+        return mrf_fail(MRF_E_ARG, "NATOMS (%zu) must be a positive multiple of natoms (%zu)",
+                        NATOMS, natoms);
+    const int n_slices = (int)(NATOMS / natoms);
+    if (echo != kEpgSsfp && n_slices != 1)
+        return mrf_fail(MRF_E_ARG, "%s fills one B1 slice only; streamed matching needs "
+                        "NATOMS == natoms", echo_type);
+    if (int rc = check_device()) return rc;
+    if (V == 0) return MRF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int T = (int)nreps;
+    // block: whole multiples of 128 atoms per slice, at most 1 GiB of atoms
+    size_t block = block_atoms;
+    if (block == 0) {
+        const size_t per = (size_t)n_slices * T * sizeof(float2);
+        block = ((size_t)1 << 30) / per / 128 * 128;
+        if (block < 128) block = 128;
+    }
+    if (block > natoms) block = natoms;
+    const int normalize = (opts && opts->normalize >= 0)
+                              ? opts->normalize
+                              : (echo == kEpgSsfp || echo == kRoaSsfp);
+    std::vector<float> seq;
+    prepare_sequence(echo, h_mrf, nreps, NREPS, seq);
+    bool h_real = true;
+    for (size_t i = 1; i < nreps; i++)
+        if (seq[nreps + i] != seq[nreps]) h_real = false;
+
+    MatchLaunch l{};
+    l.params = d_params;
+    l.sig = reinterpret_cast<const float2 *>(d_sig);
+    l.natoms = natoms;
+    l.n_slices = n_slices;
+    l.T = T;
+    l.V = V;
+    l.mode = MRF_MATCH_FP32;
+    l.maps = d_maps;
+    l.idx = d_idx;
+    const size_t nblk = (size_t)n_slices * block;
+    float *d_seq = nullptr, *d_pblk = nullptr;
+    TrCoef *d_coef = nullptr;
+    float2 *d_buf = nullptr;
+    double *d_n2 = nullptr;
+    int32_t *d_vs = nullptr;
+    void *scratch = nullptr;
+    int rc = 0;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (rc) return;
+        hipError_t e = hipMallocAsync(p, bytes, s);
+        if (e != hipSuccess)
+            rc = mrf_fail(MRF_E_NOMEM, "device allocation of %zu bytes: %s", bytes,
+                          hipGetErrorString(e));
+    };
+    alloc((void **)&d_seq, 4 * nreps * sizeof(float));
+    alloc((void **)&d_coef, (size_t)n_slices * T * sizeof(TrCoef));
+    alloc((void **)&d_buf, nblk * T * sizeof(float2));
+    alloc((void **)&d_pblk, nblk * 4 * sizeof(float));
+    if (normalize) alloc((void **)&d_n2, nblk * sizeof(double));
+    if (d_b1map) alloc((void **)&d_vs, V * sizeof(int32_t));
+    if (!rc) {
+        l.vox_slice = d_vs;
+        alloc(&scratch, match_scratch_bytes(l));
+    }
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(d_seq, seq.data(), 4 * nreps * sizeof(float),
+                                      hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = mrf_fail(MRF_E_GPU, "sequence upload: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        coef_kernel<<<dim3((T + 127) / 128), 128, 0, s>>>(d_seq, T, d_params, natoms, n_slices,
+                                                           echo == kEpgSsfp ? 1 : 0, d_coef);
+        if (d_b1map) rc = launch_b1_select(d_b1map, d_params, natoms, n_slices, V, d_vs, s);
+    }
+    if (!rc) {
+        // block a0: parameter rows a0 .. a0+nb-1 of every slice, then their atoms
+        auto gen = [&](size_t a0, size_t nb) -> int {
+            hipError_t e = hipMemcpy2DAsync(d_pblk, nb * 4 * sizeof(float), d_params + a0 * 4,
+                                            natoms * 4 * sizeof(float), nb * 4 * sizeof(float),
+                                            (size_t)n_slices, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess)
+                return mrf_fail(MRF_E_GPU, "parameter gather: %s", hipGetErrorString(e));
+            return gen_atoms(echo, d_coef, d_pblk, nb, n_slices, T, nstates, opts, h_real,
+                             normalize, d_buf, d_n2, s);
+        };
+        rc = launch_match_blocks(l, block, d_buf, scratch, s, gen);
+    }
+    for (void *p : {(void *)d_seq, (void *)d_coef, (void *)d_buf, (void *)d_pblk, (void *)d_n2,
+                    (void *)d_vs, scratch})
+        if (p) (void)hipFreeAsync(p, s);
     return rc;
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 
 #include "internal.h"
 
@@ -65,6 +66,13 @@ struct MatchLaunch {
 // Scratch size in bytes a matching call needs.
 size_t match_scratch_bytes(const MatchLaunch &a);
 int launch_match(const MatchLaunch &a, void *scratch, hipStream_t s);
+// FP32 matching of a dictionary that exists one block at a time: for each
+// block, gen(a0, nb) fills `buf` [n_slices][nb][T] with atoms a0 .. a0+nb-1
+// of every slice (nb <= block), which is then matched; a.atoms is unused.
+// The keys merge across blocks, so the maps equal launch_match's on the
+// whole dictionary.  Scratch as match_scratch_bytes(a).
+int launch_match_blocks(const MatchLaunch &a, size_t block, float2 *buf, void *scratch,
+                        hipStream_t s, const std::function<int(size_t, size_t)> &gen);
 
 // Row stride (bf16x2 words) of the low-precision copies: T rounded up to a
 // multiple of 32, the pad zero-filled.

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <type_traits>
 
 namespace mrf {
@@ -55,8 +56,11 @@ struct MatchArgs {
     const Tile *tiles;
     const int *ntiles;
     unsigned long long *keys;   // [V]
-    size_t natoms;              // atoms per slice
+    size_t natoms;              // atoms per slice in this launch
     size_t chunk;               // atoms per work-group (multiple of AT)
+    size_t row_stride;          // atom rows per slice in `atoms`
+    size_t gidx_stride;         // global atom index step per slice
+    size_t gidx_base;           // global index of atom 0 of slice 0
     int T;
 };
 
@@ -92,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
     const size_t a_begin = (size_t)blockIdx.y * a.chunk;
     if (a_begin >= a.natoms) return;
     const size_t a_end = (a_begin + a.chunk < a.natoms) ? a_begin + a.chunk : a.natoms;
-    const size_t slice_base = (size_t)tile.slice * a.natoms;
+    const size_t row_base = (size_t)tile.slice * a.row_stride;
+    const size_t idx_base = a.gidx_base + (size_t)tile.slice * a.gidx_stride;
 
     __shared__ __attribute__((aligned(16))) float lds_a[AT * LS];
     __shared__ __attribute__((aligned(16))) float lds_b[VT * LS];
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const size_t ar = sub + st_row + 32 * p;
-            arow[p] = (ar < a_end) ? a.atoms + (slice_base + ar) * (size_t)a.T : nullptr;
+            arow[p] = (ar < a_end) ? a.atoms + (row_base + ar) * (size_t)a.T : nullptr;
         }
         floatx4 acc_re[4][4], acc_im[4][4];
 #pragma unroll
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
                         const float sc = re * re + im * im;
                         if (sc > best[v]) {
                             best[v] = sc;
-                            bidx[v] = (uint32_t)(slice_base + atom);
+                            bidx[v] = (uint32_t)(idx_base + atom);
                         }
                     }
                 }
@@ -921,9 +926,11 @@ int launch_b1_select(const float *b1map, const float *params, size_t natoms, int
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "b1 select: %s", hipGetErrorString(e));
 }
 
-int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
+namespace {
+
+// Argument checks shared by the matching launches.
+int match_check(const MatchLaunch &l)
 {
-    if (l.V == 0) return 0;
     const bool lowp = l.mode != MRF_MATCH_FP32;
     if (l.mode < MRF_MATCH_FP32 || l.mode > MRF_MATCH_SPLIT2)
         return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", l.mode);
@@ -938,6 +945,13 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     if ((size_t)ns * nat > 0xfffffffeULL)
         return mrf_fail(MRF_E_ARG, "dictionary larger than 2^32-1 atoms");
     if (l.V > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many voxels");
+    return 0;
+}
+
+// Slice bucketing of the voxels and zeroed keys.
+Scratch match_setup(const MatchLaunch &l, void *scratch, hipStream_t s)
+{
+    const int ns = l.vox_slice ? l.n_slices : 1;
     Scratch sc = carve(scratch, l);
     const unsigned vb = (unsigned)((l.V + 255) / 256);
     zero_u64_kernel<<<vb, 256, 0, s>>>(sc.keys, l.V);
@@ -948,7 +962,48 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     tiles_kernel<<<1, 1, 0, s>>>(sc.counts, ns, l.mode == MRF_MATCH_FP32 ? VT : LVT, sc.offsets,
                                  sc.tiles, sc.ntiles);
     scatter_kernel<<<vb, 256, 0, s>>>(l.vox_slice, l.V, ns, sc.offsets, sc.fill, sc.perm);
+    return sc;
+}
 
+// One FP32 launch over atoms [0, nat) of every slice of `atoms` (rows
+// row_stride apart per slice); keys take global index
+// gbase + slice * gstride + atom.
+void launch_fp32(const MatchLaunch &l, const Scratch &sc, const float2 *atoms, size_t nat,
+                 size_t row_stride, size_t gstride, size_t gbase, hipStream_t s)
+{
+    MatchArgs a{};
+    a.atoms = atoms;
+    a.sig = l.sig;
+    a.perm = sc.perm;
+    a.tiles = sc.tiles;
+    a.ntiles = sc.ntiles;
+    a.keys = sc.keys;
+    a.natoms = nat;
+    a.row_stride = row_stride;
+    a.gidx_stride = gstride;
+    a.gidx_base = gbase;
+    a.T = l.T;
+    // at most kAtomChunk atoms per work-group; small problems split the
+    // atoms further so that there are about 2048 work-groups
+    size_t want = (2048 + sc.max_tiles - 1) / sc.max_tiles;
+    size_t chunk = (nat + want - 1) / want;
+    chunk = (chunk + AT - 1) / AT * AT;
+    if (chunk > (size_t)kAtomChunk) chunk = kAtomChunk;
+    a.chunk = chunk;
+    const size_t chunks = (nat + chunk - 1) / chunk;
+    dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
+    match_fp32_kernel<<<grid, kBlock, 0, s>>>(a);
+}
+
+}  // namespace
+
+int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
+{
+    if (l.V == 0) return 0;
+    if (int rc = match_check(l)) return rc;
+    const size_t nat = l.vox_slice ? l.natoms : l.natoms * (size_t)l.n_slices;
+    Scratch sc = match_setup(l, scratch, s);
+    const unsigned vb = (unsigned)((l.V + 255) / 256);
     if (l.mode != MRF_MATCH_FP32) {
         const int ld = (int)lowp_ld(l.T);
         if (int rc = launch_lowp_prepare(l.sig, sc.perm, l.V, l.T, ld, sc.shi,
@@ -997,26 +1052,38 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }
 
-    MatchArgs a{};
-    a.atoms = l.atoms;
-    a.sig = l.sig;
-    a.perm = sc.perm;
-    a.tiles = sc.tiles;
-    a.ntiles = sc.ntiles;
-    a.keys = sc.keys;
-    a.natoms = nat;
-    a.T = l.T;
-    // at most kAtomChunk atoms per work-group; small problems split the
-    // atoms further so that there are about 2048 work-groups
-    size_t want = (2048 + sc.max_tiles - 1) / sc.max_tiles;
-    size_t chunk = (nat + want - 1) / want;
-    chunk = (chunk + AT - 1) / AT * AT;
-    if (chunk > (size_t)kAtomChunk) chunk = kAtomChunk;
-    a.chunk = chunk;
-    const size_t chunks = (nat + chunk - 1) / chunk;
-    if (chunks > 65535) return mrf_fail(MRF_E_ARG, "dictionary slice too large");
-    dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
-    match_fp32_kernel<<<grid, kBlock, 0, s>>>(a);
+    if ((nat + kAtomChunk - 1) / kAtomChunk > 65535)
+        return mrf_fail(MRF_E_ARG, "dictionary slice too large");
+    launch_fp32(l, sc, l.atoms, nat, nat, nat, 0, s);
+    maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
+}
+
+int launch_match_blocks(const MatchLaunch &l, size_t block, float2 *buf, void *scratch,
+                        hipStream_t s, const std::function<int(size_t, size_t)> &gen)
+{
+    if (l.mode != MRF_MATCH_FP32)
+        return mrf_fail(MRF_E_UNSUPPORTED, "block-streamed matching is FP32 only");
+    if (block == 0) return mrf_fail(MRF_E_ARG, "block size must be positive");
+    if (l.V == 0) return 0;
+    if (int rc = match_check(l)) return rc;
+    if ((block + kAtomChunk - 1) / kAtomChunk > 65535)
+        return mrf_fail(MRF_E_ARG, "block too large");
+    Scratch sc = match_setup(l, scratch, s);
+    for (size_t a0 = 0; a0 < l.natoms; a0 += block) {
+        const size_t nb = (l.natoms - a0 < block) ? l.natoms - a0 : block;
+        if (int rc = gen(a0, nb)) return rc;
+        if (l.vox_slice) {
+            launch_fp32(l, sc, buf, nb, nb, l.natoms, a0, s);
+        } else {
+            // every voxel meets every slice: one launch per slice
+            for (int k = 0; k < l.n_slices; k++)
+                launch_fp32(l, sc, buf + (size_t)k * nb * l.T, nb, 0, 0,
+                            (size_t)k * l.natoms + a0, s);
+        }
+    }
+    const unsigned vb = (unsigned)((l.V + 255) / 256);
     maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));

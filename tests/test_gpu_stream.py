@@ -1,0 +1,91 @@
+"""Dictionary generation streamed through the FP32 matcher
+(mrf_dict_match_ex): the dictionary exists one block of atoms per slice at a
+time, and the maps and indices must equal generating the whole dictionary
+(mrf_dict_ex) and matching it (mrf_match_ex, FP32) bit for bit, for any block
+size, with and without a B1 map.  The whole-dictionary path is pinned to the
+reference's fixtures and the FP64 oracle by test_gpu_epg.py and
+test_gpu_match.py."""
+import numpy as np
+import pytest
+
+import snapmrf_amd as sm
+from snapmrf_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+T, W = 200, 12
+GRID = ("100:40:2000+2300:300:5000", "20:10:100+110:30:300+400:400:2000", "-30:6:30", "0.8:0.2:1.2")
+
+
+def _inputs(torch, echo="epg_ssfp", grid=GRID, V=3000, seed=0):
+    raw = synth.csv_columns(synth.sequence(T, seed=seed), echo)
+    params, A, natoms = sm.build_params(*grid)
+    d_params = torch.from_numpy(params).cuda()
+    d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
+    sm.dict_ex(echo, raw, T, d_params, A, natoms, W, d_atoms)
+    rng = np.random.default_rng(seed + 1)
+    ns = A // natoms
+    b1 = rng.uniform(0.75, 1.25, V).astype(np.float32)
+    slice_b1 = params[::natoms, 3]
+    ks = np.argmin(np.abs(b1[:, None] - slice_b1[None, :]), axis=1)
+    truth = ks * natoms + rng.integers(0, natoms, V)
+    sig = d_atoms[torch.from_numpy(truth).cuda()] * torch.from_numpy(
+        rng.uniform(0.3, 1.0, V).astype(np.float32)).cuda()[:, None]
+    g = torch.Generator(device="cuda").manual_seed(seed + 2)
+    sig = (sig + 0.01 * torch.complex(torch.randn(V, T, device="cuda", generator=g),
+                                      torch.randn(V, T, device="cuda", generator=g))).contiguous()
+    torch.cuda.synchronize()
+    return raw, params, d_params, A, natoms, d_atoms, sig, torch.from_numpy(b1).cuda(), ns
+
+
+def _whole(torch, d_atoms, d_params, A, natoms, sig, d_b1):
+    V = sig.shape[0]
+    d_maps = torch.empty((V, 5), device="cuda")
+    d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+    sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T, A, natoms, sig, V, d_maps,
+                d_b1map=d_b1, d_idx=d_idx)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+def _streamed(torch, echo, raw, d_params, A, natoms, sig, d_b1, block):
+    V = sig.shape[0]
+    d_maps = torch.full((V, 5), -7.0, device="cuda")
+    d_idx = torch.full((V,), -7, dtype=torch.int32, device="cuda")
+    sm.dict_match_ex(echo, raw, T, d_params, A, natoms, W, sig, V, d_maps, d_b1map=d_b1,
+                     d_idx=d_idx, block_atoms=block)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+@pytest.mark.parametrize("use_b1", [True, False])
+def test_streamed_equals_whole_dictionary(gpu, use_b1):
+    raw, params, d_params, A, natoms, d_atoms, sig, d_b1, ns = _inputs(gpu)
+    assert ns == 3 and natoms > 1000
+    b1 = d_b1 if use_b1 else None
+    m_ref, i_ref = _whole(gpu, d_atoms, d_params, A, natoms, sig, b1)
+    # 128: many blocks; 300: blocks not a multiple of the 128-atom tile;
+    # natoms: one block; 0: the automatic size
+    for block in (128, 300, natoms, 0):
+        m, i = _streamed(gpu, "epg_ssfp", raw, d_params, A, natoms, sig, b1, block)
+        assert np.array_equal(i, i_ref), block
+        assert np.array_equal(m, m_ref), block
+
+
+def test_streamed_fisp_one_slice(gpu):
+    grid = ("100:20:2000", "20:10:300", "0", "1")
+    raw, params, d_params, A, natoms, d_atoms, sig, d_b1, ns = _inputs(gpu, "epg_fisp", grid, V=1500)
+    m_ref, i_ref = _whole(gpu, d_atoms, d_params, A, natoms, sig, None)
+    m, i = _streamed(gpu, "epg_fisp", raw, d_params, A, natoms, sig, None, 512)
+    assert np.array_equal(i, i_ref) and np.array_equal(m, m_ref)
+
+
+def test_streamed_rejects_partial_dictionaries(gpu):
+    raw = synth.csv_columns(synth.sequence(T, seed=0), "roa_ssfp")
+    params, A, natoms = sm.build_params(*GRID)
+    d_params = gpu.from_numpy(params).cuda()
+    sig = gpu.zeros((4, T), dtype=gpu.complex64, device="cuda")
+    d_maps = gpu.empty((4, 5), device="cuda")
+    # roa_ssfp fills one slice only, so a 3-slice dictionary cannot stream
+    with pytest.raises(sm.MRFError):
+        sm.dict_match_ex("roa_ssfp", raw, T, d_params, A, natoms, W, sig, 4, d_maps)
