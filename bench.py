@@ -512,6 +512,41 @@ def roa_iso(sm, torch, np, synth, T, stream, timed, check):
     return res
 
 
+def svd_compressed(sm, torch, np, T, d_atoms, d_params, A, natoms, d_sig, d_b1, V, idx32, truth,
+                   match_ms, timed, stream, energy=0.99999, k_max=256):
+    """Rank-K compressed matching (per-B1-slice SVD bases, the MATLAB
+    energy rule at `energy`) of the same voxels: basis, dictionary
+    compression and compressed matching times, and agreement with the
+    uncompressed FP32 matcher."""
+    ns = A // natoms
+    d_basis = torch.empty((ns * k_max, T), dtype=torch.complex64, device="cuda")
+    t0 = time.time()
+    K, sv = sm.svd_basis(d_atoms, T, A, natoms, d_basis, k_max, True, energy, stream=stream)
+    basis_s = time.time() - t0
+    d_cat = torch.empty((A, K), dtype=torch.complex64, device="cuda")
+    comp_ms = timed(lambda: sm.svd_compress(d_basis, K, T, d_atoms, A, natoms, d_cat,
+                                            stream=stream), 1)
+    d_maps = torch.empty((V, 5), device="cuda")
+    d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+
+    def run():
+        sm.svd_match_ex(d_basis, K, T, d_cat, d_params, A, natoms, d_sig, V, d_maps,
+                        d_b1map=d_b1, d_idx=d_idx, stream=stream)
+    run()
+    torch.cuda.synchronize()
+    ms = timed(run, 2)
+    i = d_idx.cpu().numpy()
+    res = {"energy_frac": energy, "K": K, "basis_s": basis_s, "compress_dictionary_ms": comp_ms,
+           "match_ms": ms, "voxels_per_s": V / (ms * 1e-3),
+           "speedup_vs_fp32_match": match_ms / ms,
+           "argmax_agree_fp32": float(np.mean(i == idx32)),
+           "truth_agree": float(np.mean(i == truth))}
+    log(f"SVD-compressed matching: K={K}, basis {basis_s:.2f} s, compress {comp_ms:.1f} ms, "
+        f"match {ms:.1f} ms, agree fp32 {res['argmax_agree_fp32']:.4f}")
+    del d_basis, d_cat, d_maps, d_idx
+    return res
+
+
 def measured_traffic(key: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
     summary (profiles/traffic.json, FETCH_SIZE + WRITE_SIZE scaled by the
@@ -664,6 +699,9 @@ def main() -> int:
         log(f"streamed generation + matching: {st_ms:.1f} ms "
             f"(equal: {out['streamed_gen_match']['idx_equal_whole_dictionary']})")
         del d_maps_s, d_idx_s
+        out["svd_compressed"] = svd_compressed(sm, torch, np, T, d_atoms, d_params, A, natoms,
+                                               d_sig, d_b1, V, idx, truth, match_ms, timed,
+                                               stream)
     if not args.skip_diff:
         out["differentiation"] = differentiation(sm, torch, np, raw, T, w, d_atoms, d_params,
                                                  params, A, natoms, stream, timed,

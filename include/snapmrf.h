@@ -189,6 +189,37 @@ int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
                       const float *d_b1map, size_t block_atoms, float *d_maps,
                       int32_t *d_idx);
 
+/* Rank-K (SVD) compressed matching, the dictionary compression of McGivney
+ * et al. (IEEE TMI 2014) as matlab/MRF_dict_B1.m:125-159 builds it; the
+ * reference's CUDA path has no counterpart.
+ *
+ * mrf_svd_basis: left singular vectors of the (normalised) atoms with the
+ * largest singular values, per B1 slice (per_slice = 1; NATOMS/natoms bases)
+ * or for the whole dictionary (0), from the FP64 Gram matrix and FP64
+ * eigensolver.  Rank: with energy_frac > 0, the number of singular values
+ * whose cumulative energy fraction is <= energy_frac on the first basis (the
+ * MATLAB rule), at least 1 and at most K_max; else K_max.  d_basis receives
+ * [bases][K][T] complex64 (allocate bases * K_max * T), h_sv (optional,
+ * host) [bases][K_max] singular values, *K_out the rank K.  Synchronous.
+ *
+ * mrf_svd_compress: d_catoms [NATOMS][K] = U^H * atoms, each slice with its
+ * basis.
+ *
+ * mrf_svd_match_ex: compresses each signal with its B1 slice's basis (the
+ * slice the B1 map selects, as mrf_match_ex) and runs the FP32 matcher over
+ * the K compressed values; d_maps / d_idx as mrf_match_ex.  Per-slice bases
+ * need d_b1map. */
+int mrf_svd_basis(void *stream, const mrf_complex *d_atoms, size_t T, size_t NATOMS,
+                  size_t natoms, int per_slice, double energy_frac, size_t K_max,
+                  mrf_complex *d_basis, double *h_sv, size_t *K_out);
+int mrf_svd_compress(void *stream, const mrf_complex *d_basis, size_t K, size_t T,
+                     int per_slice, const mrf_complex *d_atoms, size_t NATOMS, size_t natoms,
+                     mrf_complex *d_catoms);
+int mrf_svd_match_ex(void *stream, const mrf_complex *d_basis, size_t K, size_t T,
+                     int per_slice, const mrf_complex *d_catoms, const float *d_params,
+                     size_t NATOMS, size_t natoms, const mrf_complex *d_sig, size_t V,
+                     const float *d_b1map, float *d_maps, int32_t *d_idx);
+
 /* Builds the BF16 copies of a complex64 dictionary: d_hi = bf16(a),
  * d_lo = bf16(a - hi) (optional), round to nearest even.  Each is
  * [NATOMS][mrf_lowp_ld(T)] words of two bf16 (re in the low half, im in the

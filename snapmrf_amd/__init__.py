@@ -30,7 +30,8 @@ EXPORTS = (
     "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
     "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
     "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
-    "mrf_match_ex", "mrf_dict_match_ex", "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
+    "mrf_match_ex", "mrf_dict_match_ex", "mrf_svd_basis", "mrf_svd_compress", "mrf_svd_match_ex",
+    "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
     "mrf_gn_refine", "mrf_crb_loss_grad", "mrf_device_count",
 )
 
@@ -85,6 +86,9 @@ def lib() -> ctypes.CDLL:
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
         "mrf_dict_match_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, S, P,
                                S, P, P], I),
+        "mrf_svd_basis": ([P, P, S, S, S, I, ctypes.c_double, S, P, P, ctypes.POINTER(S)], I),
+        "mrf_svd_compress": ([P, P, S, S, I, P, S, S, P], I),
+        "mrf_svd_match_ex": ([P, P, S, S, I, P, P, S, S, P, S, P, P, P], I),
         "mrf_lowp_prepare": ([P, P, S, S, P, P], I),
         "mrf_lowp_ld": ([S], S),
         "mrf_epg_jacobian": ([P, P, S, S, P, S, I, P, P], I),
@@ -300,6 +304,35 @@ def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, nato
                                   h.shape[1], _ptr(d_params), NATOMS, natoms, nstates,
                                   ctypes.byref(opts), _ptr(d_sig), V, _ptr(d_b1map), block_atoms,
                                   _ptr(d_maps), _ptr(d_idx)))
+
+
+def svd_basis(d_atoms, T: int, NATOMS: int, natoms: int, d_basis, K_max: int,
+              per_slice: bool = True, energy_frac: float = 0.0, stream=None):
+    """Rank-K basis (McGivney SVD compression); d_basis device complex64 with
+    room for bases * K_max * T values.  Returns (K, singular values
+    [bases, K_max] float64, zero past K)."""
+    import numpy as np
+    nb = NATOMS // natoms if per_slice else 1
+    sv = np.zeros((nb, K_max), dtype=np.float64)
+    K = ctypes.c_size_t()
+    check(lib().mrf_svd_basis(_stream_ptr(stream), _ptr(d_atoms), T, NATOMS, natoms,
+                              1 if per_slice else 0, energy_frac, K_max, _ptr(d_basis),
+                              sv.ctypes.data, ctypes.byref(K)))
+    return int(K.value), sv
+
+
+def svd_compress(d_basis, K: int, T: int, d_atoms, NATOMS: int, natoms: int, d_catoms,
+                 per_slice: bool = True, stream=None) -> None:
+    check(lib().mrf_svd_compress(_stream_ptr(stream), _ptr(d_basis), K, T, 1 if per_slice else 0,
+                                 _ptr(d_atoms), NATOMS, natoms, _ptr(d_catoms)))
+
+
+def svd_match_ex(d_basis, K: int, T: int, d_catoms, d_params, NATOMS: int, natoms: int, d_sig,
+                 V: int, d_maps, d_b1map=None, d_idx=None, per_slice: bool = True,
+                 stream=None) -> None:
+    check(lib().mrf_svd_match_ex(_stream_ptr(stream), _ptr(d_basis), K, T, 1 if per_slice else 0,
+                                 _ptr(d_catoms), _ptr(d_params), NATOMS, natoms, _ptr(d_sig), V,
+                                 _ptr(d_b1map), _ptr(d_maps), _ptr(d_idx)))
 
 
 def lowp_ld(T: int) -> int:

@@ -23,9 +23,11 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsnapmrf.so")
 CLI = os.path.join(HERE, "bin", "mrf")
 
-HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "grad.hip", "capi.hip"]
+HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "grad.hip", "svd.hip", "capi.hip"]
 HOST_SOURCES = ["host.cpp"]
 ARCH = "gfx950"
+# rocBLAS / rocSOLVER: the plain GEMMs and the eigensolver of the SVD basis
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _hipcc() -> str:
@@ -73,7 +75,7 @@ def build(verbose: bool = False) -> str:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-Wl,-soname,libsnapmrf.so"])
+          "-Wl,-soname,libsnapmrf.so", *LIBS])
     os.replace(tmp, LIB)
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
     cli_obj = _compile_cli()
