@@ -157,6 +157,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     for name, mode, k in (("fp32", sm.MRF_MATCH_FP32, 1),
                           ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
                           ("bf16_top16", sm.MRF_MATCH_BF16_TOP16, steps),
+                          ("bf16_top32", sm.MRF_MATCH_BF16_TOP32, steps),
                           ("split2", sm.MRF_MATCH_SPLIT2, steps),
                           ("split3", sm.MRF_MATCH_SPLIT3, steps)):
         d_maps = torch.empty((V, 5), device="cuda")
@@ -175,7 +176,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                      "truth_agree": float(np.mean(out[name][1] == truth))}
         log(f"precision form {name}: {ms:.1f} ms/call")
     i32 = out["fp32"][1]
-    lowp_names = ("bf16_top8", "bf16_top16", "split2", "split3")
+    lowp_names = ("bf16_top8", "bf16_top16", "bf16_top32", "split2", "split3")
     for name in lowp_names:
         res[name]["argmax_agree_fp32"] = float(np.mean(out[name][1] == i32))
         res[name]["speedup_vs_fp32"] = res[name]["voxels_per_s"] / res["fp32"]["voxels_per_s"]

@@ -153,6 +153,7 @@ int mrf_match(float *d_maps, const mrf_complex *h_img,
 #define MRF_MATCH_SPLIT3 2      /* split-BF16 3 products (hi*hi + lo*hi + hi*lo) + top-8 + rescore */
 #define MRF_MATCH_BF16_TOP16 3  /* BF16 MFMA + top-16 + rescore */
 #define MRF_MATCH_SPLIT2 4      /* dictionary split (a_hi + a_lo) * s_hi + top-8 + rescore */
+#define MRF_MATCH_BF16_TOP32 5  /* BF16 MFMA + 24 candidates per chunk + top-32 + rescore */
 
 /* Extended matching, all device-resident, on a caller stream.
  *   d_atoms   [NATOMS][T] complex64 (used by FP32 and for the rescore)

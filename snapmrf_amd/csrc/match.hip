@@ -405,8 +405,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LAT = 128;          // atoms per work-group sub-tile
 constexpr int LVT = 256;          // voxels per work-group tile (64 per wave)
-constexpr int kLaneK = 4;         // candidates per lane and voxel
-constexpr int kChunkK = 16;       // candidates per voxel and atom chunk (4 lanes)
+// LK candidates per lane and voxel (4, or 6 for BF16_TOP32); a voxel's four
+// lane lists make its 4*LK candidates per atom chunk.
 constexpr int kMaxChunks = 32;    // rescore merges up to 512 entries per voxel
 constexpr int kMinChunkAtoms = 4 * LAT;
 constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
@@ -417,8 +417,8 @@ struct LowpArgs {
     const int32_t *perm;
     const Tile *tiles;
     const int *ntiles;
-    float *part_s;               // [chunks][V][kChunkK] scores (-1 = empty)
-    uint32_t *part_i;            // [chunks][V][kChunkK] global atom indices
+    float *part_s;               // [chunks][V][4 * LK] scores (-1 = empty)
+    uint32_t *part_i;            // [chunks][V][4 * LK] global atom indices
     size_t natoms;               // atoms per slice
     size_t chunk;                // atoms per work-group
     size_t V;
@@ -451,27 +451,34 @@ __device__ __forceinline__ u32x4 bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
     return __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
 }
 
-// Insert (sc, gi) into the sorted 4-entry list at ls/li (score descending;
+// Insert (sc, gi) into the sorted LK-entry list at ls/li (score descending;
 // equal scores keep the earlier, lower index first), given that sc beats the
 // last entry.  Branch-free; returns the new last score.
+template <int LK>
 __device__ __forceinline__ float lane_insert(float *ls, uint32_t *li, float sc, uint32_t gi)
 {
-    const float4 S = *reinterpret_cast<const float4 *>(ls);
-    const uint4 I = *reinterpret_cast<const uint4 *>(li);
-    const bool b0 = S.x >= sc, b1 = S.y >= sc, b2 = S.z >= sc;
-    float4 So;
-    uint4 Io;
-    So.x = b0 ? S.x : sc;
-    Io.x = b0 ? I.x : gi;
-    So.y = b1 ? S.y : (b0 ? sc : S.x);
-    Io.y = b1 ? I.y : (b0 ? gi : I.x);
-    So.z = b2 ? S.z : (b1 ? sc : S.y);
-    Io.z = b2 ? I.z : (b1 ? gi : I.y);
-    So.w = b2 ? sc : S.z;
-    Io.w = b2 ? gi : I.z;
-    *reinterpret_cast<float4 *>(ls) = So;
-    *reinterpret_cast<uint4 *>(li) = Io;
-    return So.w;
+    float S[LK], So[LK];
+    uint32_t I[LK], Io[LK];
+    bool keep[LK];
+#pragma unroll
+    for (int j = 0; j < LK; j++) {
+        S[j] = ls[j];
+        I[j] = li[j];
+        keep[j] = S[j] >= sc;
+    }
+    So[0] = keep[0] ? S[0] : sc;
+    Io[0] = keep[0] ? I[0] : gi;
+#pragma unroll
+    for (int j = 1; j < LK; j++) {
+        So[j] = keep[j] ? S[j] : (keep[j - 1] ? sc : S[j - 1]);
+        Io[j] = keep[j] ? I[j] : (keep[j - 1] ? gi : I[j - 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < LK; j++) {
+        ls[j] = So[j];
+        li[j] = Io[j];
+    }
+    return So[LK - 1];
 }
 
 __device__ __forceinline__ floatx4 mfma_bf16(u32x4 a, u32x4 b, floatx4 c)
@@ -486,8 +493,9 @@ __device__ __forceinline__ int row_atom(int s)
     return ((s >> 2) & 3) + 4 * (s & 3) + (s & ~15);
 }
 
-// KC timepoints per K chunk (16 or 32), two LDS buffers.
-template <int NPROD, int KC>
+// KC timepoints per K chunk (16 or 32), two LDS buffers; LK candidates per
+// lane and voxel.
+template <int NPROD, int KC, int LK>
 __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
 {
     constexpr bool ASPLIT = NPROD >= 2;
@@ -501,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
     constexpr int NB = 2;
     static_assert(KC == 16 || KC == 32, "K chunk");
     static_assert(4 * (NB * (BA * (ASPLIT ? 2 : 1) + BX * (SSPLIT ? 2 : 1)) +
-                       2 * 4 * LVT * kLaneK) <= 160 * 1024,
+                       2 * 4 * LVT * LK) <= 160 * 1024,
                   "LDS budget");
 
     // XCD-aware numbering: work-group L runs on XCD L % 8
@@ -521,8 +529,8 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
     __shared__ __attribute__((aligned(16))) uint32_t lAl[ASPLIT ? NB * BA : 4];
     __shared__ __attribute__((aligned(16))) uint32_t lX[NB * BX];
     __shared__ __attribute__((aligned(16))) uint32_t lXl[SSPLIT ? NB * BX : 4];
-    __shared__ __attribute__((aligned(16))) float cs[4 * LVT * kLaneK];
-    __shared__ __attribute__((aligned(16))) uint32_t ci[4 * LVT * kLaneK];
+    __shared__ __attribute__((aligned(16))) float cs[4 * LVT * LK];
+    __shared__ __attribute__((aligned(16))) uint32_t ci[4 * LVT * LK];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -532,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
     const int st_row = tid / SEGS;
     const uint32_t st_b = (uint32_t)(tid % SEGS) * 16u;
 
-    for (int q = tid; q < 4 * LVT * kLaneK; q += kBlock) {
+    for (int q = tid; q < 4 * LVT * LK; q += kBlock) {
         cs[q] = -1.0f;
         ci[q] = 0xffffffffu;
     }
@@ -685,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         const int nvalid = (int)nsub;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int q = (g * LVT + w * 64 + v * 16 + r16) * kLaneK;
+            const int q = (g * LVT + w * 64 + v * 16 + r16) * LK;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 float sc[16];
@@ -706,23 +714,27 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
 #pragma unroll
                     for (int k = 1; k < 16; k++) s1 = (j == k) ? sc[k] : s1;
                     if (s1 > th[v])
-                        th[v] = lane_insert(&cs[q], &ci[q], s1,
+                        th[v] = lane_insert<LK>(&cs[q], &ci[q], s1,
                                             lane_atom0 + 4u * (j & 3) + 16u * (4 * h + (j >> 2)));
                 }
             }
         }
     }
     __syncthreads();
-    // the 4 lane lists of each voxel are its 16 candidates of this chunk
+    // the 4 lane lists of each voxel are its 4 * LK candidates of this chunk
     if (tid < tile.count) {
+        constexpr int CK = 4 * LK;
         const size_t vox = (size_t)a.perm[tile.pstart + tid];
-        float4 *ps = reinterpret_cast<float4 *>(a.part_s + ((size_t)chunk_id * a.V + vox) * kChunkK);
-        uint4 *pi = reinterpret_cast<uint4 *>(a.part_i + ((size_t)chunk_id * a.V + vox) * kChunkK);
+        float2 *ps = reinterpret_cast<float2 *>(a.part_s + ((size_t)chunk_id * a.V + vox) * CK);
+        uint2 *pi = reinterpret_cast<uint2 *>(a.part_i + ((size_t)chunk_id * a.V + vox) * CK);
 #pragma unroll
         for (int gg = 0; gg < 4; gg++) {
-            const int q = (gg * LVT + tid) * kLaneK;
-            ps[gg] = *reinterpret_cast<const float4 *>(&cs[q]);
-            pi[gg] = *reinterpret_cast<const uint4 *>(&ci[q]);
+            const int q = (gg * LVT + tid) * LK;
+#pragma unroll
+            for (int j = 0; j < LK / 2; j++) {
+                ps[gg * (LK / 2) + j] = *reinterpret_cast<const float2 *>(&cs[q + 2 * j]);
+                pi[gg * (LK / 2) + j] = *reinterpret_cast<const uint2 *>(&ci[q + 2 * j]);
+            }
         }
     }
 }
@@ -735,18 +747,19 @@ __device__ __forceinline__ bool cand_better(float s1, uint32_t i1, float s2, uin
 // Per voxel (one wave): merge the chunk candidate lists, keep the top K by
 // low-precision score, rescore them (FP64 accumulation) from the complex64
 // dictionary and keep the best (first index on ties).
-template <int K>
+template <int K, int CK>
 __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
                                                          const uint32_t *part_i, int nchunks,
                                                          const float2 *atoms, const float2 *sig,
                                                          const float *params, size_t V, int T,
                                                          float *maps, int32_t *idx)
 {
-    constexpr int PER = kMaxChunks * kChunkK / 64;   // entries per lane
+    constexpr int PER = kMaxChunks * CK / 64;   // entries per lane
+    static_assert(kMaxChunks * CK % 64 == 0, "candidate entries per lane");
     const size_t vox = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (vox >= V) return;
-    const int n = nchunks * kChunkK;
+    const int n = nchunks * CK;
     float s[PER];
     uint32_t id[PER];
 #pragma unroll
@@ -755,8 +768,8 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
         s[q] = -1.0f;
         id[q] = 0xffffffffu;
         if (e < n) {
-            const int ch = e / kChunkK, k = e % kChunkK;
-            const size_t o = ((size_t)ch * V + vox) * kChunkK + k;
+            const int ch = e / CK, k = e % CK;
+            const size_t o = ((size_t)ch * V + vox) * CK + k;
             s[q] = part_s[o];
             id[q] = part_i[o];
         }
@@ -789,38 +802,46 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
     }
     // Rescore with FP64 accumulation: products of two floats are exact in
     // double, so the candidates are ranked by their (nearly) exact scores
-    // and equal scores resolve to the lowest index.
-    double re[K], im[K];
-#pragma unroll
-    for (int c = 0; c < K; c++) re[c] = im[c] = 0.0;
-    const float2 *x = sig + vox * (size_t)T;
-    for (int t = lane; t < T; t += 64) {
-        const float2 xv = x[t];
-#pragma unroll
-        for (int c = 0; c < K; c++)
-            if (c < nc) {
-                const float2 av = atoms[(size_t)cand[c] * T + t];
-                re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
-                im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
-            }
-    }
-#pragma unroll
-    for (int c = 0; c < K; c++)
-        for (int off = 32; off >= 1; off >>= 1) {
-            re[c] += __shfl_xor(re[c], off);
-            im[c] += __shfl_xor(im[c], off);
-        }
+    // and equal scores resolve to the lowest index.  Candidates go in groups
+    // of at most 16 (the signal row is re-read from cache per group), which
+    // keeps the accumulators within 64 registers.
+    constexpr int G = K < 16 ? K : 16;
     double bs = -1.0;
     uint32_t bi = 0u;
+    const float2 *x = sig + vox * (size_t)T;
 #pragma unroll
-    for (int c = 0; c < K; c++)
-        if (c < nc) {
-            const double sc = re[c] * re[c] + im[c] * im[c];
-            if (sc > bs || (sc == bs && cand[c] < bi)) {
-                bs = sc;
-                bi = cand[c];
-            }
+    for (int c0 = 0; c0 < K; c0 += G) {
+        if (c0 >= nc) break;
+        double re[G], im[G];
+#pragma unroll
+        for (int c = 0; c < G; c++) re[c] = im[c] = 0.0;
+        for (int t = lane; t < T; t += 64) {
+            const float2 xv = x[t];
+#pragma unroll
+            for (int c = 0; c < G; c++)
+                if (c0 + c < nc) {
+                    const float2 av = atoms[(size_t)cand[c0 + c] * T + t];
+                    re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
+                    im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
+                }
         }
+#pragma unroll
+        for (int c = 0; c < G; c++)
+            for (int off = 32; off >= 1; off >>= 1) {
+                re[c] += __shfl_xor(re[c], off);
+                im[c] += __shfl_xor(im[c], off);
+            }
+#pragma unroll
+        for (int c = 0; c < G; c++)
+            if (c0 + c < nc) {
+                const double sc = re[c] * re[c] + im[c] * im[c];
+                const uint32_t ci = cand[c0 + c];
+                if (sc > bs || (sc == bs && ci < bi)) {
+                    bs = sc;
+                    bi = ci;
+                }
+            }
+    }
     if (nc == 0) {
         bs = 0.0;
         bi = 0u;
@@ -856,6 +877,12 @@ struct Scratch {
 // work-groups of one XCD share a voxel tile), enough for about 2048
 // work-groups, at most kMaxChunks; chunks hold whole sub-tiles and at least
 // kMinChunkAtoms atoms.
+// candidates per lane and voxel of a low-precision mode
+int lane_k(int mode)
+{
+    return mode == MRF_MATCH_BF16_TOP32 ? 6 : 4;
+}
+
 void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
 {
     size_t want = (2048 + max_tiles - 1) / max_tiles;
@@ -901,9 +928,10 @@ Scratch carve(void *base, const MatchLaunch &l)
             p += align256(l.V * ld * 4);
         }
         s.part_s = (float *)p;
-        p += align256(s.nchunks * l.V * kChunkK * 4);
+        const size_t ck = 4 * (size_t)lane_k(l.mode);
+        p += align256(s.nchunks * l.V * ck * 4);
         s.part_i = (uint32_t *)p;
-        p += align256(s.nchunks * l.V * kChunkK * 4);
+        p += align256(s.nchunks * l.V * ck * 4);
     }
     s.bytes = (size_t)(p - (uint8_t *)base);
     return s;
@@ -932,7 +960,7 @@ namespace {
 int match_check(const MatchLaunch &l)
 {
     const bool lowp = l.mode != MRF_MATCH_FP32;
-    if (l.mode < MRF_MATCH_FP32 || l.mode > MRF_MATCH_SPLIT2)
+    if (l.mode < MRF_MATCH_FP32 || l.mode > MRF_MATCH_BF16_TOP32)
         return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", l.mode);
     const bool need_lo = l.mode == MRF_MATCH_SPLIT3 || l.mode == MRF_MATCH_SPLIT2;
     if (lowp && (!l.lowp_hi || (need_lo && !l.lowp_lo)))
@@ -1033,21 +1061,30 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         // buffer, which measured 2-8% slower).
         if (l.mode == MRF_MATCH_SPLIT3) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<3, 16><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<3, 16, 4><<<grid, kBlock, 0, s>>>(b);
         } else if (l.mode == MRF_MATCH_SPLIT2) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<2, 16><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<2, 16, 4><<<grid, kBlock, 0, s>>>(b);
+        } else if (l.mode == MRF_MATCH_BF16_TOP32) {
+            b.nk = (l.T + 31) / 32;
+            match_lowp_kernel<1, 32, 6><<<grid, kBlock, 0, s>>>(b);
         } else {
             b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<1, 32, 4><<<grid, kBlock, 0, s>>>(b);
         }
         const dim3 rg((unsigned)((l.V + 3) / 4));
-        if (l.mode == MRF_MATCH_BF16_TOP16)
-            rescore_kernel<16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks, l.atoms,
-                                                     l.sig, l.params, l.V, l.T, l.maps, l.idx);
+        if (l.mode == MRF_MATCH_BF16_TOP32)
+            rescore_kernel<32, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
+                                                         l.atoms, l.sig, l.params, l.V, l.T, l.maps,
+                                                         l.idx);
+        else if (l.mode == MRF_MATCH_BF16_TOP16)
+            rescore_kernel<16, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
+                                                         l.atoms, l.sig, l.params, l.V, l.T, l.maps,
+                                                         l.idx);
         else
-            rescore_kernel<8><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks, l.atoms,
-                                                    l.sig, l.params, l.V, l.T, l.maps, l.idx);
+            rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
+                                                        l.atoms, l.sig, l.params, l.V, l.T, l.maps,
+                                                        l.idx);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }

@@ -166,8 +166,8 @@ def test_pd_merge_is_global_first_index_argmax(gpu):
 
 # --------------------------------------------------------------- BF16 ----
 
-LOWP_MODES = (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_SPLIT2,
-              sm.MRF_MATCH_SPLIT3)
+LOWP_MODES = (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_BF16_TOP32,
+              sm.MRF_MATCH_SPLIT2, sm.MRF_MATCH_SPLIT3)
 SPLIT_MODES = (sm.MRF_MATCH_SPLIT2, sm.MRF_MATCH_SPLIT3)
 
 
@@ -256,6 +256,30 @@ def test_lowp_b1_map_mode(gpu):
     mlo, ilo = match_lowp(gpu, atoms, params, sig, natoms, sm.MRF_MATCH_BF16_TOP8, b1map=b1)
     assert np.mean(ilo == i32) >= 0.999
     assert np.all(ilo // natoms == ks)
+
+
+def test_bf16_top32_candidates_contain_top16(gpu):
+    """BF16_TOP32 keeps 6 candidates per lane list (TOP16: 4) and rescores
+    the best 32 (TOP16: 16), so its candidate set contains TOP16's and its
+    FP64-rescored pick scores at least as high on every voxel; on a dense
+    grid (FP64 top-2 gaps below the BF16 rounding) it finds the FP64 argmax
+    at least as often as TOP16 and TOP8."""
+    T = 400
+    raw = synth.csv_columns(synth.sequence(T, 4), "epg_ssfp")
+    params, A, natoms = sm.build_params("300:10:900", "40:4:120", "-10:2:10", "1")
+    d_atoms = gpu.zeros((A, T), dtype=gpu.complex64, device="cuda")
+    sm.dict_ex("epg_ssfp", raw, T, dev(gpu, params), A, natoms, 30, d_atoms)
+    atoms = d_atoms.cpu().numpy()
+    sig, truth, pd = synth.signals_np(atoms, 2000, seed=21, snr=25.0)
+    ref_idx, ref_pd, ref_sec = oracle.match_slice(atoms, sig)
+    picks = {}
+    for mode in (sm.MRF_MATCH_BF16_TOP8, sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_BF16_TOP32):
+        picks[mode] = match_lowp(gpu, atoms, params, sig, natoms, mode)[1]
+    right = {m: np.mean(i == ref_idx) for m, i in picks.items()}
+    assert right[sm.MRF_MATCH_BF16_TOP32] >= right[sm.MRF_MATCH_BF16_TOP16] >= right[sm.MRF_MATCH_BF16_TOP8]
+    s16 = fp64_score(atoms, sig, picks[sm.MRF_MATCH_BF16_TOP16])
+    s32 = fp64_score(atoms, sig, picks[sm.MRF_MATCH_BF16_TOP32])
+    assert np.all(s32 >= s16 * (1 - 1e-12))
 
 
 def test_split_bf16_reference_kat(gpu, golden):
