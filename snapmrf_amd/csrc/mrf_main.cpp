@@ -9,7 +9,10 @@
 // reference's help text (--maps, --states, --timepoints, --NATOMS); -G/-B are
 // accepted and ignored (launch shapes are chosen per kernel); the image may
 // have any number of dimensions >= 2 (voxels = product of dims[1:]); errors
-// return an exit status instead of blocking on stdin.
+// return an exit status instead of blocking on stdin.  Additions: --precision
+// (BF16 / split-BF16 matchers), --stream (generation streamed through the
+// FP32 matcher, no dictionary in memory) and --svd FRAC (rank-K compressed
+// matching, the MATLAB path's SVD compression).
 #include "snapmrf.h"
 
 #include <hip/hip_runtime_api.h>
@@ -45,6 +48,8 @@ void print_usage()
     std::fprintf(stderr, "\t-G, --gridsize n\t\t set GPU gridsize (accepted, unused)\n");
     std::fprintf(stderr, "\t-B, --blocksize n\t\t set GPU blocksize (accepted, unused)\n");
     std::fprintf(stderr, "\t    --precision fp32|bf16|bf16_top16|bf16_top32|split2|split3\t matcher precision (default fp32)\n");
+    std::fprintf(stderr, "\t    --stream\t\t\t generate and match block by block (no -a)\n");
+    std::fprintf(stderr, "\t    --svd FRAC\t\t\t match in the rank-K SVD space keeping energy fraction FRAC\n");
     std::fprintf(stderr, "\t-h\t\t\t\t show this help\n");
 }
 
@@ -52,6 +57,32 @@ int fail(int status, const char *what)
 {
     std::fprintf(stderr, "mrf: %s: %s\n", what, mrf_last_error());
     return status;
+}
+
+// device copies of the image [nvoxels][nt] and of the B1 map file (optional)
+int upload_inputs(const mrf_ra_t &img, size_t nvoxels, size_t nt, const char *in_b1map,
+                  mrf_complex **d_sig, float **d_b1)
+{
+    *d_sig = nullptr;
+    *d_b1 = nullptr;
+    if (hipMalloc((void **)d_sig, nvoxels * nt * sizeof(mrf_complex)) != hipSuccess ||
+        hipMemcpy(*d_sig, img.data, nvoxels * nt * sizeof(mrf_complex),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EX_OSERR, "image upload");
+    if (!in_b1map) return 0;
+    mrf_ra_t rb;
+    if (mrf_ra_read(&rb, in_b1map)) return fail(EX_NOINPUT, "b1map");
+    if (rb.size < nvoxels * sizeof(float)) {
+        mrf_ra_free(&rb);
+        std::fprintf(stderr, "mrf: %s holds fewer than %zu floats\n", in_b1map, nvoxels);
+        return EX_DATAERR;
+    }
+    int rc = 0;
+    if (hipMalloc((void **)d_b1, nvoxels * sizeof(float)) != hipSuccess ||
+        hipMemcpy(*d_b1, rb.data, nvoxels * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(EX_OSERR, "b1map upload");
+    mrf_ra_free(&rb);
+    return rc;
 }
 
 double now()
@@ -84,6 +115,8 @@ int main(int argc, char *argv[])
     const char *T1 = "20:20:3000", *T2 = "20:10:1000", *B0 = "0", *B1 = "1";
     const char *echo_type = "epg_ssfp";
     const char *precision = "fp32";
+    bool stream = false;
+    double svd_frac = 0.0;
     int nstates = 101;
     size_t nreps = 1500, NATOMS = 0;
 
@@ -106,6 +139,8 @@ int main(int argc, char *argv[])
         {"gridsize", required_argument, nullptr, 'G'},
         {"blocksize", required_argument, nullptr, 'B'},
         {"precision", required_argument, nullptr, 'P'},
+        {"stream", no_argument, nullptr, 'S'},
+        {"svd", required_argument, nullptr, 'V'},
         {"help", no_argument, nullptr, 'h'},
         {nullptr, 0, nullptr, 0}};
     opterr = 0;
@@ -128,6 +163,8 @@ int main(int argc, char *argv[])
         case 'G':
         case 'B': break;
         case 'P': precision = optarg; break;
+        case 'S': stream = true; break;
+        case 'V': svd_frac = std::atof(optarg); break;
         case 'h':
         default: print_usage(); return 1;
         }
@@ -149,6 +186,19 @@ int main(int argc, char *argv[])
     else if (!std::strcmp(precision, "split3")) mode = MRF_MATCH_SPLIT3;
     else if (std::strcmp(precision, "fp32")) {
         std::fprintf(stderr, "mrf: unknown precision '%s'\n", precision);
+        return EX_USAGE;
+    }
+
+    if ((stream || svd_frac > 0.0) && mode != MRF_MATCH_FP32) {
+        std::fprintf(stderr, "mrf: --stream and --svd use the FP32 matcher\n");
+        return EX_USAGE;
+    }
+    if (stream && (svd_frac > 0.0 || out_atoms || !in_img)) {
+        std::fprintf(stderr, "mrf: --stream needs an image and excludes -a and --svd\n");
+        return EX_USAGE;
+    }
+    if (svd_frac < 0.0 || svd_frac > 1.0) {
+        std::fprintf(stderr, "mrf: --svd takes an energy fraction in (0, 1]\n");
         return EX_USAGE;
     }
 
@@ -208,18 +258,23 @@ int main(int argc, char *argv[])
     float *d_params = nullptr;
     mrf_complex *d_atoms = nullptr;
     if (hipMalloc((void **)&d_params, NATOMS * 4 * sizeof(float)) != hipSuccess ||
-        hipMalloc((void **)&d_atoms, nreps * NATOMS * sizeof(mrf_complex)) != hipSuccess) {
+        (!stream &&
+         hipMalloc((void **)&d_atoms, nreps * NATOMS * sizeof(mrf_complex)) != hipSuccess)) {
         std::fprintf(stderr, "mrf: device allocation failed\n");
         return EX_OSERR;
     }
-    hipMemcpy(d_params, params.data(), NATOMS * 4 * sizeof(float), hipMemcpyHostToDevice);
+    if (hipMemcpy(d_params, params.data(), NATOMS * 4 * sizeof(float), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return fail(EX_OSERR, "parameter upload");
 
-    std::printf("Constructing dictionary...\n");
     double t0 = now();
-    if (int rc = mrf_dict(d_atoms, d_params, h_mrf.data(), nreps, NREPS, NATOMS, natoms, 4,
-                          nstates, echo_type))
-        return fail(rc == MRF_E_ECHO ? EX_USAGE : EX_SOFTWARE, "dictionary");
-    std::printf("Elapsed dictionary time: %.2f s\n\n", now() - t0);
+    if (!stream) {
+        std::printf("Constructing dictionary...\n");
+        if (int rc = mrf_dict(d_atoms, d_params, h_mrf.data(), nreps, NREPS, NATOMS, natoms, 4,
+                              nstates, echo_type))
+            return fail(rc == MRF_E_ECHO ? EX_USAGE : EX_SOFTWARE, "dictionary");
+        std::printf("Elapsed dictionary time: %.2f s\n\n", now() - t0);
+    }
 
     if (!in_img && out_maps) std::printf("NO data specified! Abort matching!\n");
     if (in_img) {
@@ -241,64 +296,91 @@ int main(int argc, char *argv[])
         }
         std::printf("Matching...\n");
         t0 = now();
-        if (mode == MRF_MATCH_FP32) {
+        if (mode == MRF_MATCH_FP32 && !stream && svd_frac <= 0.0) {
             if (mrf_match(d_maps, (const mrf_complex *)img.data, d_atoms, d_params, nreps, nt,
                           NATOMS, natoms, nvoxels, 4, 5, in_b1map))
                 return fail(EX_SOFTWARE, "matching");
         } else {
-            // low-precision matcher through the extended entry points
+            // extended entry points: device-resident signals and B1 map
             if (nt != nreps) {
                 std::fprintf(stderr, "mrf: image frames (%zu) must equal nreps (%zu)\n", nt, nreps);
                 return EX_DATAERR;
             }
-            void *hi = nullptr, *lo = nullptr;
             mrf_complex *d_sig = nullptr;
             float *d_b1 = nullptr;
-            const size_t lowp_bytes = NATOMS * mrf_lowp_ld(nreps) * 4;
-            const bool need_lo = mode == MRF_MATCH_SPLIT3 || mode == MRF_MATCH_SPLIT2;
-            if (hipMalloc(&hi, lowp_bytes) != hipSuccess ||
-                (need_lo && hipMalloc(&lo, lowp_bytes) != hipSuccess) ||
-                hipMalloc((void **)&d_sig, nvoxels * nt * sizeof(mrf_complex)) != hipSuccess)
-                return fail(EX_SOFTWARE, "device allocation");
-            hipMemcpy(d_sig, img.data, nvoxels * nt * sizeof(mrf_complex), hipMemcpyHostToDevice);
-            if (in_b1map) {
-                mrf_ra_t rb;
-                if (mrf_ra_read(&rb, in_b1map)) return fail(EX_NOINPUT, "b1map");
-                hipMalloc((void **)&d_b1, nvoxels * sizeof(float));
-                hipMemcpy(d_b1, rb.data, nvoxels * sizeof(float), hipMemcpyHostToDevice);
-                mrf_ra_free(&rb);
+            if (int rc = upload_inputs(img, nvoxels, nt, in_b1map, &d_sig, &d_b1)) return rc;
+            if (stream) {
+                std::printf("Constructing and matching the dictionary block by block...\n");
+                if (int rc = mrf_dict_match_ex(nullptr, echo_type, h_mrf.data(), nreps, NREPS,
+                                               d_params, NATOMS, natoms, nstates, nullptr, d_sig,
+                                               nvoxels, d_b1, 0, d_maps, nullptr))
+                    return fail(rc == MRF_E_ECHO ? EX_USAGE : EX_SOFTWARE, "streamed matching");
+            } else if (svd_frac > 0.0) {
+                // per-slice bases when the B1 map picks each voxel's slice
+                const int per_slice = (d_b1 && NATOMS / natoms > 1) ? 1 : 0;
+                const size_t nb = per_slice ? NATOMS / natoms : 1;
+                const size_t kmax = nreps < 256 ? nreps : 256;
+                mrf_complex *basis = nullptr, *catoms = nullptr;
+                size_t K = 0;
+                if (hipMalloc((void **)&basis, nb * kmax * nreps * sizeof(mrf_complex)) != hipSuccess)
+                    return fail(EX_OSERR, "device allocation");
+                if (mrf_svd_basis(nullptr, d_atoms, nreps, NATOMS, natoms, per_slice, svd_frac,
+                                  kmax, basis, nullptr, &K))
+                    return fail(EX_SOFTWARE, "SVD basis");
+                std::printf("SVD rank: %zu (%s)\n", K, per_slice ? "per B1 slice" : "one basis");
+                if (hipMalloc((void **)&catoms, NATOMS * K * sizeof(mrf_complex)) != hipSuccess)
+                    return fail(EX_OSERR, "device allocation");
+                if (mrf_svd_compress(nullptr, basis, K, nreps, per_slice, d_atoms, NATOMS, natoms,
+                                     catoms) ||
+                    mrf_svd_match_ex(nullptr, basis, K, nreps, per_slice, catoms, d_params, NATOMS,
+                                     natoms, d_sig, nvoxels, d_b1, d_maps, nullptr))
+                    return fail(EX_SOFTWARE, "SVD matching");
+                (void)hipDeviceSynchronize();
+                (void)hipFree(basis);
+                (void)hipFree(catoms);
+            } else {
+                void *hi = nullptr, *lo = nullptr;
+                const size_t lowp_bytes = NATOMS * mrf_lowp_ld(nreps) * 4;
+                const bool need_lo = mode == MRF_MATCH_SPLIT3 || mode == MRF_MATCH_SPLIT2;
+                if (hipMalloc(&hi, lowp_bytes) != hipSuccess ||
+                    (need_lo && hipMalloc(&lo, lowp_bytes) != hipSuccess))
+                    return fail(EX_OSERR, "device allocation");
+                if (mrf_lowp_prepare(nullptr, d_atoms, nreps, NATOMS, hi, lo) ||
+                    mrf_match_ex(nullptr, mode, d_atoms, hi, lo, d_params, nreps, NATOMS, natoms,
+                                 d_sig, nvoxels, d_b1, d_maps, nullptr))
+                    return fail(EX_SOFTWARE, "matching");
+                (void)hipDeviceSynchronize();
+                (void)hipFree(hi);
+                if (lo) (void)hipFree(lo);
             }
-            if (mrf_lowp_prepare(nullptr, d_atoms, nreps, NATOMS, hi, lo) ||
-                mrf_match_ex(nullptr, mode, d_atoms, hi, lo, d_params, nreps, NATOMS, natoms,
-                             d_sig, nvoxels, d_b1, d_maps, nullptr))
-                return fail(EX_SOFTWARE, "matching");
-            hipDeviceSynchronize();
-            hipFree(hi);
-            if (lo) hipFree(lo);
-            hipFree(d_sig);
-            if (d_b1) hipFree(d_b1);
+            if (hipDeviceSynchronize() != hipSuccess) return fail(EX_SOFTWARE, "matching");
+            (void)hipFree(d_sig);
+            if (d_b1) (void)hipFree(d_b1);
         }
         std::printf("Elapsed matching time: %.2f s\n\n", now() - t0);
         if (!out_maps) out_maps = "../result/maps.ra";
         std::printf("Output maps: %s\n", out_maps);
         std::vector<float> h_map(5 * nvoxels);
-        hipMemcpy(h_map.data(), d_maps, 5 * nvoxels * sizeof(float), hipMemcpyDeviceToHost);
+        if (hipMemcpy(h_map.data(), d_maps, 5 * nvoxels * sizeof(float),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(EX_OSERR, "maps download");
         if (save_ra(h_map.data(), 3, 4, 5, nvoxels, out_maps)) return fail(EX_CANTCREAT, "maps");
-        hipFree(d_maps);
+        (void)hipFree(d_maps);
         mrf_ra_free(&img);
     }
     if (out_atoms) {
         std::printf("Output atoms: %s\n", out_atoms);
         std::vector<mrf_complex> h_atoms(nreps * NATOMS);
-        hipMemcpy(h_atoms.data(), d_atoms, nreps * NATOMS * sizeof(mrf_complex),
-                  hipMemcpyDeviceToHost);
+        if (hipMemcpy(h_atoms.data(), d_atoms, nreps * NATOMS * sizeof(mrf_complex),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(EX_OSERR, "atoms download");
         if (save_ra(h_atoms.data(), 4, 8, nreps, NATOMS, out_atoms)) return fail(EX_CANTCREAT, "atoms");
     }
     if (out_params) {
         std::printf("Output params: %s\n", out_params);
         if (save_ra(params.data(), 3, 4, 4, NATOMS, out_params)) return fail(EX_CANTCREAT, "params");
     }
-    hipFree(d_atoms);
-    hipFree(d_params);
+    if (d_atoms) (void)hipFree(d_atoms);
+    (void)hipFree(d_params);
     return 0;
 }

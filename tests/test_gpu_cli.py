@@ -132,3 +132,36 @@ def test_cli_natoms_mode_without_b1_map(gpu, tmp_path):
     clear = (rpd - rsec) / rpd > 1e-5
     assert np.all(m_cli[clear, :4] == params[ridx[clear]])
     assert np.allclose(m_cli[clear, 4], rpd[clear], rtol=1e-4)
+
+
+def test_cli_stream_equals_whole_dictionary(gpu, tmp_path):
+    """--stream generates and matches block by block: the same maps as the
+    default run, without -a."""
+    raw, params, A, natoms, atoms, sig, b1, truth, csv, img, b1f = _setup(gpu, tmp_path)
+    ref, _ = _run(tmp_path, csv, img, b1f)
+    out = {k: str(tmp_path / f"s_{k}.ra") for k in ("maps", "params")}
+    r = subprocess.run([sm.CLI_PATH, "-t", GRID[0], "-s", GRID[1], "-b", GRID[2], "-r", GRID[3],
+                        "-w", str(W), "-T", str(T), "-M", str(b1f), "-m", out["maps"],
+                        "-p", out["params"], "--stream", str(csv), str(img)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert np.array_equal(sm.ra_read(out["maps"])[0], ref["maps"][0])
+    assert np.array_equal(sm.ra_read(out["params"])[0], params)
+    # -a needs the whole dictionary
+    bad = subprocess.run([sm.CLI_PATH, "--stream", "-a", str(tmp_path / "x.ra"), str(csv), str(img)],
+                         capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0
+
+
+def test_cli_svd(gpu, tmp_path):
+    """--svd FRAC: per-B1-slice bases (the B1 map picks each voxel's basis),
+    rank by the energy rule, compressed FP32 matching."""
+    raw, params, A, natoms, atoms, sig, b1, truth, csv, img, b1f = _setup(gpu, tmp_path)
+    ref, _ = _run(tmp_path, csv, img, b1f)
+    res, stdout = _run(tmp_path, csv, img, b1f, "--svd", "0.99999")
+    assert "SVD rank:" in stdout and "per B1 slice" in stdout
+    m, m32 = res["maps"][0], ref["maps"][0]
+    # every map row is a grid point and nearly all equal the uncompressed pick
+    grid_rows = {tuple(r) for r in params.tolist()}
+    assert all(tuple(r) in grid_rows for r in m[:, :4].tolist())
+    assert np.mean(np.all(m[:, :4] == m32[:, :4], axis=1)) >= 0.95
