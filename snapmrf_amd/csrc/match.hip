@@ -99,8 +99,9 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
     const size_t row_base = (size_t)tile.slice * a.row_stride;
     const size_t idx_base = a.gidx_base + (size_t)tile.slice * a.gidx_stride;
 
-    __shared__ __attribute__((aligned(16))) float lds_a[AT * LS];
-    __shared__ __attribute__((aligned(16))) float lds_b[VT * LS];
+    // two LDS buffers: chunk c + 1 is stored while chunk c is multiplied
+    __shared__ __attribute__((aligned(16))) float lds_a[2][AT * LS];
+    __shared__ __attribute__((aligned(16))) float lds_b[2][VT * LS];
     __shared__ float red_s[2][4][VT];
     __shared__ uint32_t red_i[2][4][VT];
 
@@ -152,22 +153,25 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
             }
 
         // global -> registers (zero outside [0, T) and outside the tile); the
-        // next chunk's loads are issued before the current chunk's MFMAs
+        // next chunk's loads are issued before the current chunk's MFMAs and
+        // stored into the other LDS buffer after them (one barrier per chunk)
         float4 ga[4], gb[4];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             ga[p] = load_pair(arow[p], st_col >> 1, a.T, even_t);
             gb[p] = load_pair(vrow[p], st_col >> 1, a.T, even_t);
         }
-        for (int c = 0; c < nchunks; c++) {
-            __syncthreads();
+        // (the previous sub-tile's reads ended at its last chunk's barrier)
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                *reinterpret_cast<float4 *>(&lds_a[(st_row + 32 * p) * LS + st_col]) = ga[p];
-                *reinterpret_cast<float4 *>(&lds_b[(st_row + 32 * p) * LS + st_col]) = gb[p];
-            }
-            __syncthreads();
-            if (c + 1 < nchunks) {
+        for (int p = 0; p < 4; p++) {
+            *reinterpret_cast<float4 *>(&lds_a[0][(st_row + 32 * p) * LS + st_col]) = ga[p];
+            *reinterpret_cast<float4 *>(&lds_b[0][(st_row + 32 * p) * LS + st_col]) = gb[p];
+        }
+        __syncthreads();
+        for (int c = 0; c < nchunks; c++) {
+            const int cb = c & 1;
+            const bool more = c + 1 < nchunks;
+            if (more) {
                 const int tc = (c + 1) * KC + (st_col >> 1);
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
@@ -181,11 +185,11 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     fa[i] = *reinterpret_cast<const floatx4 *>(
-                        &lds_a[(wa * 64 + i * 16 + r16) * LS + q * 16 + g * 4]);
+                        &lds_a[cb][(wa * 64 + i * 16 + r16) * LS + q * 16 + g * 4]);
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     fx[v] = *reinterpret_cast<const floatx4 *>(
-                        &lds_b[(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
+                        &lds_b[cb][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
                     fy[v] = floatx4{fx[v][1], -fx[v][0], fx[v][3], -fx[v][2]};
                 }
 #pragma unroll
@@ -200,6 +204,14 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
                                 fa[i][s], fy[v][s], acc_im[i][v], 0, 0, 0);
                         }
             }
+            if (more) {
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    *reinterpret_cast<float4 *>(&lds_a[cb ^ 1][(st_row + 32 * p) * LS + st_col]) = ga[p];
+                    *reinterpret_cast<float4 *>(&lds_b[cb ^ 1][(st_row + 32 * p) * LS + st_col]) = gb[p];
+                }
+            }
+            __syncthreads();
         }
         // epilogue: D[row = 4g + r][col = r16] of each 16x16 tile
 #pragma unroll

@@ -85,13 +85,6 @@ rocblas_handle blas_handle(hipStream_t s, int *rc)
             return mrf_fail(MRF_E_GPU, "%s: %s", #call, rocblas_status_to_string(st_)); \
     } while (0)
 
-#define MRF_HIPC(call)                                                                  \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return mrf_fail(MRF_E_GPU, "%s: %s", #call, hipGetErrorString(e_));        \
-    } while (0)
-
 // U^H X for every slice b: X_b is [n][T] (rows), out_b [n][K]
 int compress_rows(rocblas_handle h, const float2 *basis, int K, int T, int nb,
                   const float2 *x, size_t n_per, size_t x_stride_b, float2 *out,
@@ -137,11 +130,12 @@ extern "C" int mrf_svd_basis(void *stream, const mrf_complex *d_atoms, size_t T,
     double2 *G = nullptr, *buf = nullptr;
     double *W = nullptr, *E = nullptr;
     rocblas_int *info = nullptr;
-    MRF_HIPC(hipMalloc((void **)&G, T * T * sizeof(double2)));
-    MRF_HIPC(hipMalloc((void **)&buf, chunk * T * sizeof(double2)));
-    MRF_HIPC(hipMalloc((void **)&W, T * sizeof(double)));
-    MRF_HIPC(hipMalloc((void **)&E, T * sizeof(double)));
-    MRF_HIPC(hipMalloc((void **)&info, sizeof(rocblas_int)));
+    if (hipMalloc((void **)&G, T * T * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void **)&buf, chunk * T * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void **)&W, T * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&E, T * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&info, sizeof(rocblas_int)) != hipSuccess)
+        rc = mrf_fail(MRF_E_NOMEM, "SVD basis: device allocation failed");
     std::vector<double> w(T);
     size_t K = K_max;
     for (int b = 0; b < nb && !rc; b++) {
@@ -201,11 +195,8 @@ extern "C" int mrf_svd_basis(void *stream, const mrf_complex *d_atoms, size_t T,
             G, Ti, (int)K, reinterpret_cast<float2 *>(d_basis) + (size_t)b * K * T);
     }
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = mrf_fail(MRF_E_GPU, "basis failed");
-    (void)hipFree(G);
-    (void)hipFree(buf);
-    (void)hipFree(W);
-    (void)hipFree(E);
-    (void)hipFree(info);
+    for (void *p : {(void *)G, (void *)buf, (void *)W, (void *)E, (void *)info})
+        if (p) (void)hipFree(p);
     if (!rc) *K_out = K;
     return rc;
 }
@@ -256,7 +247,8 @@ extern "C" int mrf_svd_match_ex(void *stream, const mrf_complex *d_basis, size_t
     if (!h) return rc;
     float2 *all = nullptr, *cs = nullptr;
     int32_t *vs = nullptr;
-    MRF_HIPC(hipMallocAsync((void **)&all, (size_t)nb * V * K * sizeof(float2), s));
+    if (hipMallocAsync((void **)&all, (size_t)nb * V * K * sizeof(float2), s) != hipSuccess)
+        return mrf_fail(MRF_E_NOMEM, "SVD matching: device allocation failed");
     // every basis compresses every signal; the voxel's slice picks one
     for (int b = 0; b < nb && !rc; b++)
         rc = compress_rows(h, reinterpret_cast<const float2 *>(d_basis) + (size_t)b * K * T,
