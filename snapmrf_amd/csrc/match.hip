@@ -399,16 +399,18 @@ __global__ void zero_i32_kernel(int *p, size_t n)
 // sub-tile or tile, so the staging needs no per-lane guards.
 //
 // Work-group tile: 128 atoms x 256 voxels, 4 waves of 128 atoms x 64 voxels
-// (acc 8 x 4 MFMA blocks, Re and Im); Y = (xi, -xr) is formed in registers
-// from the X fragment.  K chunks of 16 or 32 timepoints are double-buffered
+// (acc 8 x 4 MFMA blocks, Re and Im; one wave per SIMD), or with AH = 2 eight
+// waves of 64 atoms x 64 voxels (two per SIMD, one per atom half and voxel
+// quarter); Y = (xi, -xr) is formed in registers from the X fragment.  K chunks of 16 or 32 timepoints are double-buffered
 // in LDS (one barrier per chunk).  Work-groups are numbered so that the ones on
 // one XCD (work-group id mod 8) take consecutive atom chunks of the same
 // voxel tile, which then streams from that XCD's L2.
 //
 // Candidates: MFMA row 4g + r of block i holds atom sub + g + 4r + 16i, so
 // lane group g sees the atoms = g (mod 4) and neighbouring atoms (adjacent
-// T1 grid points) fall to different lanes.  Each lane keeps a sorted top-4
-// per voxel; the 4 lists of a voxel form its 16 candidates per atom chunk;
+// T1 grid points) fall to different lanes.  Each lane keeps a sorted top-LK
+// list per voxel; the 4 * AH lists of a voxel form its candidates per atom
+// chunk;
 // the rescore kernel merges the chunks, keeps the top K and rescores them with
 // FP64 sums (first index on ties, as generate_maps, src/functions.cu:1317-1349).
 // ------------------------------------------------------------------------
@@ -507,21 +509,23 @@ __device__ __forceinline__ int row_atom(int s)
 
 // KC timepoints per K chunk (16 or 32), two LDS buffers; LK candidates per
 // lane and voxel.
-template <int NPROD, int KC, int LK>
-__global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
+template <int NPROD, int KC, int LK, int AH>
+__global__ __launch_bounds__(kBlock * AH) void match_lowp_kernel(LowpArgs a)
 {
+    constexpr int NT = kBlock * AH;         // threads: AH atom halves x 4 voxel quarters
+    constexpr int NI = 8 / AH;              // 16-atom MFMA blocks per wave
     constexpr bool ASPLIT = NPROD >= 2;
     constexpr bool SSPLIT = NPROD >= 3;
     constexpr int LSB = KC + 4;             // dwords per LDS row (padded)
     constexpr int SEGS = KC / 4;            // 16-byte segments per row chunk
-    constexpr int RPP = kBlock / SEGS;      // rows staged per pass
+    constexpr int RPP = NT / SEGS;          // rows staged per pass
     constexpr int PA = LAT / RPP;
     constexpr int PX = LVT / RPP;
     constexpr int BA = LAT * LSB, BX = LVT * LSB;
     constexpr int NB = 2;
     static_assert(KC == 16 || KC == 32, "K chunk");
     static_assert(4 * (NB * (BA * (ASPLIT ? 2 : 1) + BX * (SSPLIT ? 2 : 1)) +
-                       2 * 4 * LVT * LK) <= 160 * 1024,
+                       2 * AH * 4 * LVT * LK) <= 160 * 1024,
                   "LDS budget");
 
     // XCD-aware numbering: work-group L runs on XCD L % 8
@@ -541,18 +545,19 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
     __shared__ __attribute__((aligned(16))) uint32_t lAl[ASPLIT ? NB * BA : 4];
     __shared__ __attribute__((aligned(16))) uint32_t lX[NB * BX];
     __shared__ __attribute__((aligned(16))) uint32_t lXl[SSPLIT ? NB * BX : 4];
-    __shared__ __attribute__((aligned(16))) float cs[4 * LVT * LK];
-    __shared__ __attribute__((aligned(16))) uint32_t ci[4 * LVT * LK];
+    __shared__ __attribute__((aligned(16))) float cs[AH * 4 * LVT * LK];
+    __shared__ __attribute__((aligned(16))) uint32_t ci[AH * 4 * LVT * LK];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = tid >> 6;                 // 64-voxel quarter of the tile
+    const int w = (tid >> 6) & 3;           // 64-voxel quarter of the tile
+    const int ah = tid >> 8;                // atom half (AH = 2)
     const int g = lane >> 4;
     const int r16 = lane & 15;
     const int st_row = tid / SEGS;
     const uint32_t st_b = (uint32_t)(tid % SEGS) * 16u;
 
-    for (int q = tid; q < 4 * LVT * LK; q += kBlock) {
+    for (int q = tid; q < AH * 4 * LVT * LK; q += NT) {
         cs[q] = -1.0f;
         ci[q] = 0xffffffffu;
     }
@@ -575,9 +580,9 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         __amdgpu_buffer_rsrc_t rAl = rA;
         if (ASPLIT) rAl = make_rsrc(a.alo + arow0, nsub * ldb);
 
-        floatx4 acc_re[8][4], acc_im[8][4];
+        floatx4 acc_re[NI][4], acc_im[NI][4];
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < NI; i++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -614,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         // All fragments of a chunk are read from LDS before its MFMAs, so the
         // LDS latency is paid once per chunk rather than once per A block.
         constexpr int KS = KC / 16;
-        u32x4 fx[KS][4], fxl[KS][4], fa[KS][8], fal[KS][8];
+        u32x4 fx[KS][4], fxl[KS][4], fa[KS][NI], fal[KS][NI];
         auto read_frags = [&](int b) {
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
@@ -626,8 +631,8 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                     if (SSPLIT) fxl[ks][v] = *reinterpret_cast<const u32x4 *>(&lXl[o]);
                 }
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int o = b * BA + (i * 16 + r16) * LSB + ko;
+                for (int i = 0; i < NI; i++) {
+                    const int o = b * BA + ((ah * NI + i) * 16 + r16) * LSB + ko;
                     fa[ks][i] = *reinterpret_cast<const u32x4 *>(&lA[o]);
                     if (ASPLIT) fal[ks][i] = *reinterpret_cast<const u32x4 *>(&lAl[o]);
                 }
@@ -643,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                     if (SSPLIT) fyl[v] = swap_neg4(fxl[ks][v]);
                 }
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
+                for (int i = 0; i < NI; i++) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         acc_re[i][v] = mfma_bf16(fa[ks][i], fx[ks][v], acc_re[i][v]);
@@ -670,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         // next chunk is staged into the other LDS buffer.  The global loads
         // are spread over the first half of the MFMAs and the LDS stores
         // over the second half, so the staging never holds up MFMA issue.
-        constexpr int NMF = KS * 8 * 4 * 2 * NPROD;
+        constexpr int NMF = KS * NI * 4 * 2 * NPROD;
         constexpr int NLD = PA * (ASPLIT ? 2 : 1) + PX * (SSPLIT ? 2 : 1);
         constexpr int MPL = NMF / 2 / NLD;
         // The last chunk restages itself (in bounds, unused) so that every
@@ -705,9 +710,9 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
         const int nvalid = (int)nsub;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int q = (g * LVT + w * 64 + v * 16 + r16) * LK;
+            const int q = ((ah * 4 + g) * LVT + w * 64 + v * 16 + r16) * LK;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < NI / 4; h++) {
                 float sc[16];
                 uint32_t m = 0;
 #pragma unroll
@@ -717,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                         const int i = 4 * h + ii, j = 4 * ii + r;
                         const float re = acc_re[i][v][r], im = acc_im[i][v][r];
                         sc[j] = re * re + im * im;
-                        if (g + 4 * r + 16 * i < nvalid && sc[j] > th[v]) m |= 1u << j;
+                        if (g + 4 * r + 16 * (ah * NI + i) < nvalid && sc[j] > th[v]) m |= 1u << j;
                     }
                 while (m) {
                     const int j = __builtin_ctz(m);
@@ -727,7 +732,8 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
                     for (int k = 1; k < 16; k++) s1 = (j == k) ? sc[k] : s1;
                     if (s1 > th[v])
                         th[v] = lane_insert<LK>(&cs[q], &ci[q], s1,
-                                            lane_atom0 + 4u * (j & 3) + 16u * (4 * h + (j >> 2)));
+                                            lane_atom0 + 4u * (j & 3) +
+                                                16u * (ah * NI + 4 * h + (j >> 2)));
                 }
             }
         }
@@ -735,12 +741,12 @@ __global__ __launch_bounds__(kBlock) void match_lowp_kernel(LowpArgs a)
     __syncthreads();
     // the 4 lane lists of each voxel are its 4 * LK candidates of this chunk
     if (tid < tile.count) {
-        constexpr int CK = 4 * LK;
+        constexpr int CK = 4 * AH * LK;
         const size_t vox = (size_t)a.perm[tile.pstart + tid];
         float2 *ps = reinterpret_cast<float2 *>(a.part_s + ((size_t)chunk_id * a.V + vox) * CK);
         uint2 *pi = reinterpret_cast<uint2 *>(a.part_i + ((size_t)chunk_id * a.V + vox) * CK);
 #pragma unroll
-        for (int gg = 0; gg < 4; gg++) {
+        for (int gg = 0; gg < 4 * AH; gg++) {
             const int q = (gg * LVT + tid) * LK;
 #pragma unroll
             for (int j = 0; j < LK / 2; j++) {
@@ -895,6 +901,13 @@ int lane_k(int mode)
     return mode == MRF_MATCH_BF16_TOP32 ? 6 : 4;
 }
 
+// atom halves of a low-precision mode's work-group (2: eight waves, two per
+// SIMD, each 64 atoms x 64 voxels)
+int atom_halves(int mode)
+{
+    return (mode == MRF_MATCH_SPLIT2 || mode == MRF_MATCH_BF16_TOP32) ? 2 : 1;
+}
+
 void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
 {
     size_t want = (2048 + max_tiles - 1) / max_tiles;
@@ -940,7 +953,7 @@ Scratch carve(void *base, const MatchLaunch &l)
             p += align256(l.V * ld * 4);
         }
         s.part_s = (float *)p;
-        const size_t ck = 4 * (size_t)lane_k(l.mode);
+        const size_t ck = 4 * (size_t)atom_halves(l.mode) * lane_k(l.mode);
         p += align256(s.nchunks * l.V * ck * 4);
         s.part_i = (uint32_t *)p;
         p += align256(s.nchunks * l.V * ck * 4);
@@ -1067,36 +1080,41 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         const size_t nwg = (sc.max_tiles * sc.nchunks + 7) / 8 * 8;
         if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
         const dim3 grid((unsigned)nwg);
-        // Measured on the 446k-atom x 460,800-voxel workload: 32-timepoint
-        // chunks in two LDS buffers for one product (7% faster than 16);
-        // the split forms keep 16-timepoint chunks (32 fits only one LDS
-        // buffer, which measured 2-8% slower).
+        // Measured on the 446k-atom x 460,800-voxel workload.  One product:
+        // 32-timepoint chunks, one wave per SIMD (7 % faster than 16-timepoint
+        // chunks; two waves per SIMD need 16-timepoint chunks to fit the LDS
+        // and measured 2 % slower).  SPLIT2 and BF16_TOP32: two waves per SIMD
+        // (64-atom wave tiles), 6-13 % faster, and twice the candidates per
+        // chunk.  SPLIT3: one wave per SIMD (two do not fit the LDS).
+        const int nc = (int)sc.nchunks;
+        const dim3 rg((unsigned)((l.V + 3) / 4));
         if (l.mode == MRF_MATCH_SPLIT3) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<3, 16, 4><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<3, 16, 4, 1><<<grid, kBlock, 0, s>>>(b);
+            rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
+                                                        l.params, l.V, l.T, l.maps, l.idx);
         } else if (l.mode == MRF_MATCH_SPLIT2) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<2, 16, 4><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<2, 16, 4, 2><<<grid, 2 * kBlock, 0, s>>>(b);
+            rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
+                                                        l.params, l.V, l.T, l.maps, l.idx);
         } else if (l.mode == MRF_MATCH_BF16_TOP32) {
-            b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32, 6><<<grid, kBlock, 0, s>>>(b);
+            b.nk = (l.T + 15) / 16;
+            match_lowp_kernel<1, 16, 6, 2><<<grid, 2 * kBlock, 0, s>>>(b);
+            rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
+                                                         l.params, l.V, l.T, l.maps, l.idx);
         } else {
             b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32, 4><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<1, 32, 4, 1><<<grid, kBlock, 0, s>>>(b);
+            if (l.mode == MRF_MATCH_BF16_TOP16)
+                rescore_kernel<16, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
+                                                             l.sig, l.params, l.V, l.T, l.maps,
+                                                             l.idx);
+            else
+                rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
+                                                            l.sig, l.params, l.V, l.T, l.maps,
+                                                            l.idx);
         }
-        const dim3 rg((unsigned)((l.V + 3) / 4));
-        if (l.mode == MRF_MATCH_BF16_TOP32)
-            rescore_kernel<32, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
-                                                         l.atoms, l.sig, l.params, l.V, l.T, l.maps,
-                                                         l.idx);
-        else if (l.mode == MRF_MATCH_BF16_TOP16)
-            rescore_kernel<16, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
-                                                         l.atoms, l.sig, l.params, l.V, l.T, l.maps,
-                                                         l.idx);
-        else
-            rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, (int)sc.nchunks,
-                                                        l.atoms, l.sig, l.params, l.V, l.T, l.maps,
-                                                        l.idx);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }
