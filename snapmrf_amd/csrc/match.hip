@@ -743,15 +743,15 @@ __global__ __launch_bounds__(kBlock * AH) void match_lowp_kernel(LowpArgs a)
     if (tid < tile.count) {
         constexpr int CK = 4 * AH * LK;
         const size_t vox = (size_t)a.perm[tile.pstart + tid];
-        float2 *ps = reinterpret_cast<float2 *>(a.part_s + ((size_t)chunk_id * a.V + vox) * CK);
-        uint2 *pi = reinterpret_cast<uint2 *>(a.part_i + ((size_t)chunk_id * a.V + vox) * CK);
+        float *ps = a.part_s + ((size_t)chunk_id * a.V + vox) * CK;
+        uint32_t *pi = a.part_i + ((size_t)chunk_id * a.V + vox) * CK;
 #pragma unroll
         for (int gg = 0; gg < 4 * AH; gg++) {
             const int q = (gg * LVT + tid) * LK;
 #pragma unroll
-            for (int j = 0; j < LK / 2; j++) {
-                ps[gg * (LK / 2) + j] = *reinterpret_cast<const float2 *>(&cs[q + 2 * j]);
-                pi[gg * (LK / 2) + j] = *reinterpret_cast<const uint2 *>(&ci[q + 2 * j]);
+            for (int j = 0; j < LK; j++) {
+                ps[gg * LK + j] = cs[q + j];
+                pi[gg * LK + j] = ci[q + j];
             }
         }
     }
@@ -898,14 +898,15 @@ struct Scratch {
 // candidates per lane and voxel of a low-precision mode
 int lane_k(int mode)
 {
-    return mode == MRF_MATCH_BF16_TOP32 ? 6 : 4;
+    if (mode == MRF_MATCH_BF16_TOP32) return 6;
+    return (mode == MRF_MATCH_SPLIT2 || mode == MRF_MATCH_SPLIT3) ? 4 : 3;
 }
 
 // atom halves of a low-precision mode's work-group (2: eight waves, two per
 // SIMD, each 64 atoms x 64 voxels)
 int atom_halves(int mode)
 {
-    return (mode == MRF_MATCH_SPLIT2 || mode == MRF_MATCH_BF16_TOP32) ? 2 : 1;
+    return mode == MRF_MATCH_SPLIT3 ? 1 : 2;
 }
 
 void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
@@ -1080,12 +1081,14 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         const size_t nwg = (sc.max_tiles * sc.nchunks + 7) / 8 * 8;
         if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
         const dim3 grid((unsigned)nwg);
-        // Measured on the 446k-atom x 460,800-voxel workload.  One product:
-        // 32-timepoint chunks, one wave per SIMD (7 % faster than 16-timepoint
-        // chunks; two waves per SIMD need 16-timepoint chunks to fit the LDS
-        // and measured 2 % slower).  SPLIT2 and BF16_TOP32: two waves per SIMD
-        // (64-atom wave tiles), 6-13 % faster, and twice the candidates per
-        // chunk.  SPLIT3: one wave per SIMD (two do not fit the LDS).
+        // Launch shapes measured on the 446k-atom x 460,800-voxel workload.
+        // Two waves per SIMD (AH = 2) beat one by 6-13 % wherever the LDS
+        // holds them.  TOP8/TOP16: 32-timepoint chunks with 3-entry lane
+        // lists (9 % faster than one wave per SIMD with 4-entry lists).
+        // TOP32: 6-entry lists need 16-timepoint chunks (3-entry lists at 32
+        // missed the exact pick on 0.11 % of voxels, 6-entry ones on 0.005 %).
+        // SPLIT2: 16-timepoint chunks.  SPLIT3: one wave per SIMD (two do not
+        // fit the LDS).
         const int nc = (int)sc.nchunks;
         const dim3 rg((unsigned)((l.V + 3) / 4));
         if (l.mode == MRF_MATCH_SPLIT3) {
@@ -1105,13 +1108,13 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
                                                          l.params, l.V, l.T, l.maps, l.idx);
         } else {
             b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32, 4, 1><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<1, 32, 3, 2><<<grid, 2 * kBlock, 0, s>>>(b);
             if (l.mode == MRF_MATCH_BF16_TOP16)
-                rescore_kernel<16, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
+                rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
                                                              l.sig, l.params, l.V, l.T, l.maps,
                                                              l.idx);
             else
-                rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
+                rescore_kernel<8, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
                                                             l.sig, l.params, l.V, l.T, l.maps,
                                                             l.idx);
         }
