@@ -398,10 +398,9 @@ __global__ void zero_i32_kernel(int *p, size_t n)
 // buffer loads whose range check returns zeros for rows past the end of a
 // sub-tile or tile, so the staging needs no per-lane guards.
 //
-// Work-group tile: 128 atoms x 256 voxels, 4 waves of 128 atoms x 64 voxels
-// (acc 8 x 4 MFMA blocks, Re and Im; one wave per SIMD), or with AH = 2 eight
-// waves of 64 atoms x 64 voxels (two per SIMD, one per atom half and voxel
-// quarter); Y = (xi, -xr) is formed in registers from the X fragment.  K chunks of 16 or 32 timepoints are double-buffered
+// Work-group tile: 128 atoms x 256 voxels, 8 waves of 64 atoms x 64 voxels
+// (two per SIMD, one per atom half and voxel quarter; acc 4 x 4 MFMA blocks,
+// Re and Im); Y = (xi, -xr) is formed in registers from the X fragment.  K chunks of 16 or 32 timepoints are double-buffered
 // in LDS (one barrier per chunk).  Work-groups are numbered so that the ones on
 // one XCD (work-group id mod 8) take consecutive atom chunks of the same
 // voxel tile, which then streams from that XCD's L2.
@@ -409,15 +408,17 @@ __global__ void zero_i32_kernel(int *p, size_t n)
 // Candidates: MFMA row 4g + r of block i holds atom sub + g + 4r + 16i, so
 // lane group g sees the atoms = g (mod 4) and neighbouring atoms (adjacent
 // T1 grid points) fall to different lanes.  Each lane keeps a sorted top-LK
-// list per voxel; the 4 * AH lists of a voxel form its candidates per atom
-// chunk;
-// the rescore kernel merges the chunks, keeps the top K and rescores them with
+// list per voxel; the 8 lists of a voxel (4 lanes x 2 atom halves) form its
+// candidates per atom chunk; the rescore kernel merges the chunks, keeps the
+// top K and rescores them with
 // FP64 sums (first index on ties, as generate_maps, src/functions.cu:1317-1349).
 // ------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LAT = 128;          // atoms per work-group sub-tile
+constexpr int kAtomHalves = 2;    // waves per SIMD: 64-atom wave tiles
+constexpr int kLowpThreads = 4 * 64 * kAtomHalves;
 constexpr int LVT = 256;          // voxels per work-group tile (64 per wave)
 // LK candidates per lane and voxel (4, or 6 for BF16_TOP32); a voxel's four
 // lane lists make its 4*LK candidates per atom chunk.
@@ -509,10 +510,11 @@ __device__ __forceinline__ int row_atom(int s)
 
 // KC timepoints per K chunk (16 or 32), two LDS buffers; LK candidates per
 // lane and voxel.
-template <int NPROD, int KC, int LK, int AH>
-__global__ __launch_bounds__(kBlock * AH) void match_lowp_kernel(LowpArgs a)
+template <int NPROD, int KC, int LK>
+__global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 {
-    constexpr int NT = kBlock * AH;         // threads: AH atom halves x 4 voxel quarters
+    constexpr int AH = kAtomHalves;
+    constexpr int NT = kLowpThreads;        // threads: AH atom halves x 4 voxel quarters
     constexpr int NI = 8 / AH;              // 16-atom MFMA blocks per wave
     constexpr bool ASPLIT = NPROD >= 2;
     constexpr bool SSPLIT = NPROD >= 3;
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(kBlock * AH) void match_lowp_kernel(LowpArgs a)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = (tid >> 6) & 3;           // 64-voxel quarter of the tile
-    const int ah = tid >> 8;                // atom half (AH = 2)
+    const int ah = tid >> 8;                // atom half
     const int g = lane >> 4;
     const int r16 = lane & 15;
     const int st_row = tid / SEGS;
@@ -898,16 +900,15 @@ struct Scratch {
 // candidates per lane and voxel of a low-precision mode
 int lane_k(int mode)
 {
-    if (mode == MRF_MATCH_BF16_TOP32) return 6;
-    return (mode == MRF_MATCH_SPLIT2 || mode == MRF_MATCH_SPLIT3) ? 4 : 3;
+    switch (mode) {
+    case MRF_MATCH_BF16_TOP32: return 6;
+    case MRF_MATCH_SPLIT2: return 4;
+    case MRF_MATCH_SPLIT3: return 2;
+    default: return 3;
+    }
 }
 
-// atom halves of a low-precision mode's work-group (2: eight waves, two per
-// SIMD, each 64 atoms x 64 voxels)
-int atom_halves(int mode)
-{
-    return mode == MRF_MATCH_SPLIT3 ? 1 : 2;
-}
+
 
 void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
 {
@@ -954,7 +955,7 @@ Scratch carve(void *base, const MatchLaunch &l)
             p += align256(l.V * ld * 4);
         }
         s.part_s = (float *)p;
-        const size_t ck = 4 * (size_t)atom_halves(l.mode) * lane_k(l.mode);
+        const size_t ck = 4 * (size_t)kAtomHalves * lane_k(l.mode);
         p += align256(s.nchunks * l.V * ck * 4);
         s.part_i = (uint32_t *)p;
         p += align256(s.nchunks * l.V * ck * 4);
@@ -1081,34 +1082,35 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         const size_t nwg = (sc.max_tiles * sc.nchunks + 7) / 8 * 8;
         if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
         const dim3 grid((unsigned)nwg);
-        // Launch shapes measured on the 446k-atom x 460,800-voxel workload.
-        // Two waves per SIMD (AH = 2) beat one by 6-13 % wherever the LDS
-        // holds them.  TOP8/TOP16: 32-timepoint chunks with 3-entry lane
-        // lists (9 % faster than one wave per SIMD with 4-entry lists).
-        // TOP32: 6-entry lists need 16-timepoint chunks (3-entry lists at 32
-        // missed the exact pick on 0.11 % of voxels, 6-entry ones on 0.005 %).
-        // SPLIT2: 16-timepoint chunks.  SPLIT3: one wave per SIMD (two do not
-        // fit the LDS).
+        // Launch shapes measured on the 446k-atom x 460,800-voxel workload:
+        // eight waves (two per SIMD) were 6-13 % faster than four waves of
+        // 128-atom tiles in every mode; the LDS (two staging buffers + the
+        // lane lists) sets the chunk length and list size per mode.
+        // TOP8/TOP16: 32-timepoint chunks, 3-entry lists.  TOP32: 6-entry lists
+        // need 16-timepoint chunks (3-entry lists at 32 missed the exact pick
+        // on 0.11 % of voxels, 6-entry ones on 0.005 %).  SPLIT2: 16, 4-entry.
+        // SPLIT3: 16, 2-entry.
         const int nc = (int)sc.nchunks;
         const dim3 rg((unsigned)((l.V + 3) / 4));
+        const dim3 blk(kLowpThreads);
         if (l.mode == MRF_MATCH_SPLIT3) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<3, 16, 4, 1><<<grid, kBlock, 0, s>>>(b);
+            match_lowp_kernel<3, 16, 2><<<grid, blk, 0, s>>>(b);
             rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
                                                         l.params, l.V, l.T, l.maps, l.idx);
         } else if (l.mode == MRF_MATCH_SPLIT2) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<2, 16, 4, 2><<<grid, 2 * kBlock, 0, s>>>(b);
+            match_lowp_kernel<2, 16, 4><<<grid, blk, 0, s>>>(b);
             rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
                                                         l.params, l.V, l.T, l.maps, l.idx);
         } else if (l.mode == MRF_MATCH_BF16_TOP32) {
             b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<1, 16, 6, 2><<<grid, 2 * kBlock, 0, s>>>(b);
+            match_lowp_kernel<1, 16, 6><<<grid, blk, 0, s>>>(b);
             rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
                                                          l.params, l.V, l.T, l.maps, l.idx);
         } else {
             b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32, 3, 2><<<grid, 2 * kBlock, 0, s>>>(b);
+            match_lowp_kernel<1, 32, 3><<<grid, blk, 0, s>>>(b);
             if (l.mode == MRF_MATCH_BF16_TOP16)
                 rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
                                                              l.sig, l.params, l.V, l.T, l.maps,
