@@ -87,8 +87,18 @@ __device__ __forceinline__ float4 load_pair(const float2 *row, int t, int T, boo
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
+// Eight waves: 4 atom groups of 32 x 2 voxel halves of 64 (two waves per
+// SIMD; 2 % faster than four waves of 64 x 64 at one per SIMD).
+constexpr int kFp32AtomGroups = 4;
+constexpr int kFp32Threads = 128 * kFp32AtomGroups;
+
+__global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
 {
+    constexpr int NWA = kFp32AtomGroups;
+    constexpr int NT = kFp32Threads;    // NWA atom groups x 2 voxel halves of waves
+    constexpr int NIB = AT / NWA / 16;  // 16-atom MFMA blocks per wave
+    constexpr int RP = NT / 8;          // staged rows per pass
+    constexpr int NP = AT / RP;         // passes (atoms and voxels: 128 rows each)
     const bool even_t = (a.T & 1) == 0;
     const int tile_id = blockIdx.x;
     if (tile_id >= *a.ntiles) return;
@@ -102,27 +112,27 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
     // two LDS buffers: chunk c + 1 is stored while chunk c is multiplied
     __shared__ __attribute__((aligned(16))) float lds_a[2][AT * LS];
     __shared__ __attribute__((aligned(16))) float lds_b[2][VT * LS];
-    __shared__ float red_s[2][4][VT];
-    __shared__ uint32_t red_i[2][4][VT];
+    __shared__ float red_s[NWA][4][VT];
+    __shared__ uint32_t red_i[NWA][4][VT];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wa = wave & 1;          // atom half of the sub-tile (64 atoms)
-    const int wv = wave >> 1;         // voxel half of the tile (64 voxels)
+    const int wa = wave % NWA;        // atom group of the sub-tile (AT / NWA atoms)
+    const int wv = wave / NWA;        // voxel half of the tile (64 voxels)
     const int g = lane >> 4;          // k group
     const int r16 = lane & 15;
 
-    // staging map: 8 float4 per row-chunk (KF = 32 floats); 256 threads cover
-    // 32 rows per pass, 4 passes for 128 rows.
-    const int st_row = tid >> 3;      // 0..31
+    // staging map: 8 float4 per row-chunk (KF = 32 floats); NT threads cover
+    // RP rows per pass, NP passes for 128 rows.
+    const int st_row = tid >> 3;      // 0..RP-1
     const int st_col = (tid & 7) * 4; // float offset within the chunk row
 
     // voxel rows of this tile (gathered through the permutation)
-    const float2 *vrow[4];
+    const float2 *vrow[NP];
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int vr = st_row + 32 * p;
+    for (int p = 0; p < NP; p++) {
+        const int vr = st_row + RP * p;
         vrow[p] = (vr < tile.count) ? a.sig + (size_t)a.perm[tile.pstart + vr] * (size_t)a.T
                                     : nullptr;
     }
@@ -137,15 +147,15 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
 
     const int nchunks = (a.T + KC - 1) / KC;
     for (size_t sub = a_begin; sub < a_end; sub += AT) {
-        const float2 *arow[4];
+        const float2 *arow[NP];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const size_t ar = sub + st_row + 32 * p;
+        for (int p = 0; p < NP; p++) {
+            const size_t ar = sub + st_row + RP * p;
             arow[p] = (ar < a_end) ? a.atoms + (row_base + ar) * (size_t)a.T : nullptr;
         }
-        floatx4 acc_re[4][4], acc_im[4][4];
+        floatx4 acc_re[NIB][4], acc_im[NIB][4];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NIB; i++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -155,17 +165,17 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
         // global -> registers (zero outside [0, T) and outside the tile); the
         // next chunk's loads are issued before the current chunk's MFMAs and
         // stored into the other LDS buffer after them (one barrier per chunk)
-        float4 ga[4], gb[4];
+        float4 ga[NP], gb[NP];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
+        for (int p = 0; p < NP; p++) {
             ga[p] = load_pair(arow[p], st_col >> 1, a.T, even_t);
             gb[p] = load_pair(vrow[p], st_col >> 1, a.T, even_t);
         }
         // (the previous sub-tile's reads ended at its last chunk's barrier)
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            *reinterpret_cast<float4 *>(&lds_a[0][(st_row + 32 * p) * LS + st_col]) = ga[p];
-            *reinterpret_cast<float4 *>(&lds_b[0][(st_row + 32 * p) * LS + st_col]) = gb[p];
+        for (int p = 0; p < NP; p++) {
+            *reinterpret_cast<float4 *>(&lds_a[0][(st_row + RP * p) * LS + st_col]) = ga[p];
+            *reinterpret_cast<float4 *>(&lds_b[0][(st_row + RP * p) * LS + st_col]) = gb[p];
         }
         __syncthreads();
         for (int c = 0; c < nchunks; c++) {
@@ -174,18 +184,18 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
             if (more) {
                 const int tc = (c + 1) * KC + (st_col >> 1);
 #pragma unroll
-                for (int p = 0; p < 4; p++) {
+                for (int p = 0; p < NP; p++) {
                     ga[p] = load_pair(arow[p], tc, a.T, even_t);
                     gb[p] = load_pair(vrow[p], tc, a.T, even_t);
                 }
             }
 #pragma unroll
             for (int q = 0; q < KF / 16; q++) {
-                floatx4 fa[4], fx[4], fy[4];
+                floatx4 fa[NIB], fx[4], fy[4];
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+                for (int i = 0; i < NIB; i++)
                     fa[i] = *reinterpret_cast<const floatx4 *>(
-                        &lds_a[cb][(wa * 64 + i * 16 + r16) * LS + q * 16 + g * 4]);
+                        &lds_a[cb][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     fx[v] = *reinterpret_cast<const floatx4 *>(
@@ -195,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
 #pragma unroll
                 for (int s = 0; s < 4; s++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
+                    for (int i = 0; i < NIB; i++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
                             acc_re[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
@@ -206,19 +216,19 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
             }
             if (more) {
 #pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    *reinterpret_cast<float4 *>(&lds_a[cb ^ 1][(st_row + 32 * p) * LS + st_col]) = ga[p];
-                    *reinterpret_cast<float4 *>(&lds_b[cb ^ 1][(st_row + 32 * p) * LS + st_col]) = gb[p];
+                for (int p = 0; p < NP; p++) {
+                    *reinterpret_cast<float4 *>(&lds_a[cb ^ 1][(st_row + RP * p) * LS + st_col]) = ga[p];
+                    *reinterpret_cast<float4 *>(&lds_b[cb ^ 1][(st_row + RP * p) * LS + st_col]) = gb[p];
                 }
             }
             __syncthreads();
         }
         // epilogue: D[row = 4g + r][col = r16] of each 16x16 tile
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < NIB; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const size_t atom = sub + wa * 64 + i * 16 + 4 * g + r;
+                const size_t atom = sub + wa * (AT / NWA) + i * 16 + 4 * g + r;
                 if (atom < a_end) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
@@ -233,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
             }
     }
 
-    // reduce the 8 candidates per voxel (4 k-groups x 2 atom halves)
+    // reduce the 4 * NWA candidates per voxel (4 k-groups x NWA atom groups)
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         red_s[wa][g][wv * 64 + v * 16 + r16] = best[v];
@@ -243,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void match_fp32_kernel(MatchArgs a)
     if (tid < VT && tid < tile.count) {
         float bs = -1.0f;
         uint32_t bi = 0xffffffffu;
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < NWA; h++)
             for (int q = 0; q < 4; q++) {
                 const float s2 = red_s[h][q][tid];
                 const uint32_t i2 = red_i[h][q][tid];
@@ -1047,7 +1057,7 @@ void launch_fp32(const MatchLaunch &l, const Scratch &sc, const float2 *atoms, s
     a.chunk = chunk;
     const size_t chunks = (nat + chunk - 1) / chunk;
     dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
-    match_fp32_kernel<<<grid, kBlock, 0, s>>>(a);
+    match_fp32_kernel<<<grid, kFp32Threads, 0, s>>>(a);
 }
 
 }  // namespace
