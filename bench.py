@@ -679,6 +679,24 @@ def main() -> int:
                                  "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
                                  "algorithmic_flops_per_launch": mflops}
+        # the reference interface: MRF_match with a host image and a B1-map
+        # file (PCIe upload and file read included, wall clock)
+        import tempfile
+        h_img = d_sig.cpu().numpy()
+        with tempfile.TemporaryDirectory() as tdir:
+            b1_path = os.path.join(tdir, "b1.ra")
+            sm.ra_write(b1_path, d_b1.cpu().numpy())
+            d_maps_h = torch.empty((V, 5), device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.time()
+            sm.MRF_match(d_maps_h, h_img, d_atoms, d_params, T, T, A, natoms, V, 4, 5, b1_path)
+            host_ms = (time.time() - t0) * 1e3
+        out["match_host_image"] = {
+            "ms": host_ms, "voxels_per_s": V / (host_ms * 1e-3),
+            "image_bytes": int(h_img.nbytes),
+            "maps_equal_device_path": bool(torch.equal(d_maps_h, d_maps))}
+        log(f"MRF_match with a host image: {host_ms:.1f} ms")
+        del h_img, d_maps_h
         # the same generation + matching streamed: the dictionary exists one
         # block (at most 1 GiB) at a time
         d_maps_s = torch.empty((V, 5), device="cuda")
