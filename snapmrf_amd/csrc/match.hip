@@ -845,15 +845,33 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
         double re[G], im[G];
 #pragma unroll
         for (int c = 0; c < G; c++) re[c] = im[c] = 0.0;
-        for (int t = lane; t < T; t += 64) {
-            const float2 xv = x[t];
+        if ((T & 1) == 0) {
+            // even T: rows are 16-byte aligned, two timepoints per load
+            const float4 *x4 = reinterpret_cast<const float4 *>(x);
+            for (int t = lane; t < T / 2; t += 64) {
+                const float4 xv = x4[t];
 #pragma unroll
-            for (int c = 0; c < G; c++)
-                if (c0 + c < nc) {
-                    const float2 av = atoms[(size_t)cand[c0 + c] * T + t];
-                    re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
-                    im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
-                }
+                for (int c = 0; c < G; c++)
+                    if (c0 + c < nc) {
+                        const float4 av = reinterpret_cast<const float4 *>(
+                            atoms + (size_t)cand[c0 + c] * T)[t];
+                        re[c] += (double)av.x * xv.x + (double)av.y * xv.y +
+                                 ((double)av.z * xv.z + (double)av.w * xv.w);
+                        im[c] += (double)av.x * xv.y - (double)av.y * xv.x +
+                                 ((double)av.z * xv.w - (double)av.w * xv.z);
+                    }
+            }
+        } else {
+            for (int t = lane; t < T; t += 64) {
+                const float2 xv = x[t];
+#pragma unroll
+                for (int c = 0; c < G; c++)
+                    if (c0 + c < nc) {
+                        const float2 av = atoms[(size_t)cand[c0 + c] * T + t];
+                        re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
+                        im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
+                    }
+            }
         }
 #pragma unroll
         for (int c = 0; c < G; c++)
