@@ -639,23 +639,25 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
             st_load(seg + (size_t)(t - t0) * C, w, k, yn);
             const TrK c = tr_coef(A.seq[t], p);
             float ga = 0.f, gt = 0.f;
-            // relax(TR - TE): input = dephase(phase_b(phase_a(relax_a(rf(yn)))))
+            // the TR's forward chain once: x1 = rf(yn), x2 = relax_a(x1),
+            // x3 = phase_a(x2); the reverse steps take their inputs from it
+            Aug<ND> x1 = yn;
+            f_rf(x1, c);
+            Aug<ND> x2 = x1;
+            f_relax(x2, c.e1a, c.e2a, c.te, p, k == 0);
+            Aug<ND> x3 = x2;
+            f_phase(x3, c.pa, c.te);
+            // relax(TR - TE): input = dephase(phase_b(x3))
             {
-                Aug<ND> x = yn;
-                f_rf(x, c);
-                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
-                f_phase(x, c.pa, c.te);
+                Aug<ND> x = x3;
                 f_phase(x, c.pb, c.t2);
                 f_dephase<ND, SUB>(x, k, w);
                 r_relax(l, x, c.e1b, c.e2b, c.t2, p, k == 0, &gt, gth);
             }
             r_dephase<ND, SUB>(l, k, w);
-            // phase(TR - TE) and the readout: input = phase_a(relax_a(rf(yn)))
+            // phase(TR - TE) and the readout: input = x3
             {
-                Aug<ND> x = yn;
-                f_rf(x, c);
-                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
-                f_phase(x, c.pa, c.te);
+                const Aug<ND> &x = x3;
                 r_phase(l, x, c.pb, c.t2, beta, &gt, gth);
                 if (k == 0) {
                     if constexpr (ND > 0) {
@@ -678,19 +680,10 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
                     }
                 }
             }
-            // phase(TE): input = relax_a(rf(yn))
-            {
-                Aug<ND> x = yn;
-                f_rf(x, c);
-                f_relax(x, c.e1a, c.e2a, c.te, p, k == 0);
-                r_phase(l, x, c.pa, c.te, beta, nullptr, gth);
-            }
-            // relax(TE): input = rf(yn)
-            {
-                Aug<ND> x = yn;
-                f_rf(x, c);
-                r_relax(l, x, c.e1a, c.e2a, c.te, p, k == 0, nullptr, gth);
-            }
+            // phase(TE): input = x2
+            r_phase(l, x2, c.pa, c.te, beta, nullptr, gth);
+            // relax(TE): input = x1
+            r_relax(l, x1, c.e1a, c.e2a, c.te, p, k == 0, nullptr, gth);
             r_rf(l, yn, c, p.B1, &ga, gth);
             if (A.grad_seq) {
                 ga = group_sum<SUB>(ga);
