@@ -1,7 +1,8 @@
 """FP64 CPU oracle for snapMRF's dictionary-generation and matching path.
 
-TEST INFRASTRUCTURE ONLY: the checker that tests/, ``__graft_entry__.smoke()``
-and bench.py's ``cpu_baseline`` leg compare the HIP product path against.  The
+TEST INFRASTRUCTURE ONLY: the FP64 reference that tests/,
+``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg compare the HIP
+product path against.  The
 product package (``snapmrf_amd``) never imports this module.
 
 Thin ctypes wrappers over ``oracle/liboracle.so`` (built by ``oracle/Makefile``

@@ -1,7 +1,8 @@
 /*
  * oracle.c -- FP64 CPU restatement of snapMRF's EPG / ROA dictionary and
- * matching logic.  TEST INFRASTRUCTURE ONLY (see oracle.h): the checker for
- * the HIP path and the CPU baseline that bench.py times beside it.
+ * matching logic.  TEST INFRASTRUCTURE ONLY (see oracle.h): the reference
+ * the HIP path is compared against, and the CPU baseline that bench.py times
+ * beside it.
  *
  * The step order of every program follows the reference host loops one to
  * one (file:line cited per function); each step is applied to the whole

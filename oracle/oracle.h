@@ -2,8 +2,8 @@
  * oracle.h -- FP64 CPU restatement of snapMRF's dictionary-generation and
  * template-matching path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
- * path (snapmrf_amd/libsnapmrf.so).  Only tests/, __graft_entry__.smoke() and
+ * TEST INFRASTRUCTURE ONLY.  This library is the FP64 reference the HIP
+ * product path (snapmrf_amd/libsnapmrf.so) is compared against.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it, and only to check or to time the
  * CPU baseline.  The product never links, loads or calls it.
  *
