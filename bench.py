@@ -578,8 +578,8 @@ def main() -> int:
                          "runs; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the whole path on a single "
-              "MI355X (no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the whole path runs on a single MI355X "
+              "(there is no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
