@@ -125,9 +125,9 @@ PEAK_BF16 = 2500.0         # TFLOP/s, MI355X dense BF16 MFMA
 
 
 def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle_voxels=256):
-    """BF16 storage + BF16 MFMA top-8 + FP32 rescore, and split-BF16, against
-    the FP32 matcher on 240x240x8 voxels; map errors against the FP64 oracle
-    on a seeded voxel subset."""
+    """BF16 storage + BF16 MFMA + rescore of the top 8/16/32 candidates, and
+    the split-BF16 forms, against the FP32 matcher on 240x240x8 voxels; map
+    errors against the FP64 oracle on a seeded voxel subset."""
     params, A, natoms = sm.build_params(*GRID_LOWP)
     log(f"precision form: {A} atoms ({A // natoms} slices x {natoms}), 460800 voxels")
     d_params = torch.from_numpy(params).cuda()
