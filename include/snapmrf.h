@@ -174,21 +174,25 @@ int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                  size_t natoms, const mrf_complex *d_sig, size_t V,
                  const float *d_b1map, float *d_maps, int32_t *d_idx);
 
-/* Dictionary generation streamed through the FP32 matcher: the atoms exist
- * one block at a time (block_atoms per B1 slice, every slice; 0 = blocks of
- * at most 1 GiB), each block is generated as mrf_dict_ex would generate it
- * and matched at once, and the running per-voxel (score, first index) merges
- * across blocks.  d_maps / d_idx equal mrf_dict_ex followed by mrf_match_ex
- * (MRF_MATCH_FP32) on the same inputs, without the NATOMS x T dictionary in
- * memory.  No counterpart in the reference, which always materialises the
- * dictionary (src/mrf.cu:286-321).  epg_ssfp with any number of B1 slices;
- * the other echo types need NATOMS == natoms (they fill one slice). */
+/* Dictionary generation streamed through the matcher: the atoms exist one
+ * block at a time (block_atoms per B1 slice, every slice; 0 = blocks of at
+ * most 1 GiB), each block is generated as mrf_dict_ex would generate it and
+ * matched at once, and the running per-voxel best merges across blocks.
+ * mode MRF_MATCH_FP32: d_maps / d_idx equal mrf_dict_ex followed by
+ * mrf_match_ex (MRF_MATCH_FP32) on the same inputs, without the NATOMS x T
+ * dictionary in memory.  Low-precision modes: each block is converted to
+ * BF16 and its candidates are rescored against its complex64 rows; the
+ * candidates are a superset of the whole-dictionary call's, so the pick
+ * scores at least as high in FP64.  No counterpart in the reference, which
+ * always materialises the dictionary (src/mrf.cu:286-321).  epg_ssfp with
+ * any number of B1 slices; the other echo types need NATOMS == natoms (they
+ * fill one slice). */
 int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
                       size_t nreps, size_t NREPS, const float *d_params,
                       size_t NATOMS, size_t natoms, int nstates,
-                      const mrf_dict_opts *opts, const mrf_complex *d_sig, size_t V,
-                      const float *d_b1map, size_t block_atoms, float *d_maps,
-                      int32_t *d_idx);
+                      const mrf_dict_opts *opts, int mode, const mrf_complex *d_sig,
+                      size_t V, const float *d_b1map, size_t block_atoms,
+                      float *d_maps, int32_t *d_idx);
 
 /* Rank-K (SVD) compressed matching, the dictionary compression of McGivney
  * et al. (IEEE TMI 2014) as matlab/MRF_dict_B1.m:125-159 builds it; the

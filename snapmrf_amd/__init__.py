@@ -85,8 +85,8 @@ def lib() -> ctypes.CDLL:
         "mrf_dict_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, P], I),
         "mrf_match": ([P, P, P, P, S, S, S, S, S, I, I, cs], I),
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
-        "mrf_dict_match_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, S, P,
-                               S, P, P], I),
+        "mrf_dict_match_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), I, P, S,
+                               P, S, P, P], I),
         "mrf_svd_basis": ([P, P, S, S, S, I, ctypes.c_double, S, P, P, ctypes.POINTER(S)], I),
         "mrf_svd_compress": ([P, P, S, S, I, P, S, S, P], I),
         "mrf_svd_match_ex": ([P, P, S, S, I, P, P, S, S, P, S, P, P, P], I),
@@ -291,10 +291,10 @@ def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_s
 def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int,
                   nstates: int, d_sig, V: int, d_maps, d_b1map=None, d_idx=None,
                   block_atoms: int = 0, normalize: int = -1, n_iso: int = 1,
-                  iso_spread_hz: float = 0.0, stream=None) -> None:
-    """Dictionary generation streamed through the FP32 matcher, block_atoms
-    atoms per slice at a time (0: blocks of at most 1 GiB); the maps equal
-    dict_ex + match_ex(MRF_MATCH_FP32)."""
+                  iso_spread_hz: float = 0.0, mode: int = 0, stream=None) -> None:
+    """Dictionary generation streamed through the matcher (mode MRF_MATCH_*),
+    block_atoms atoms per slice at a time (0: blocks of at most 1 GiB); with
+    FP32 the maps equal dict_ex + match_ex(MRF_MATCH_FP32)."""
     import numpy as np
     h = np.ascontiguousarray(h_mrf, dtype=np.float32)
     opts = MrfDictOpts()
@@ -303,8 +303,8 @@ def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, nato
     opts.iso_spread_hz = iso_spread_hz
     check(lib().mrf_dict_match_ex(_stream_ptr(stream), _enc(echo_type), h.ctypes.data, nreps,
                                   h.shape[1], _ptr(d_params), NATOMS, natoms, nstates,
-                                  ctypes.byref(opts), _ptr(d_sig), V, _ptr(d_b1map), block_atoms,
-                                  _ptr(d_maps), _ptr(d_idx)))
+                                  ctypes.byref(opts), mode, _ptr(d_sig), V, _ptr(d_b1map),
+                                  block_atoms, _ptr(d_maps), _ptr(d_idx)))
 
 
 def svd_basis(d_atoms, T: int, NATOMS: int, natoms: int, d_basis, K_max: int,

@@ -266,9 +266,9 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
 extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
                                  size_t nreps, size_t NREPS, const float *d_params,
                                  size_t NATOMS, size_t natoms, int nstates,
-                                 const mrf_dict_opts *opts, const mrf_complex *d_sig, size_t V,
-                                 const float *d_b1map, size_t block_atoms, float *d_maps,
-                                 int32_t *d_idx)
+                                 const mrf_dict_opts *opts, int mode, const mrf_complex *d_sig,
+                                 size_t V, const float *d_b1map, size_t block_atoms,
+                                 float *d_maps, int32_t *d_idx)
 {
     mrf_clear_error();
     const int echo = echo_id(echo_type);
@@ -313,9 +313,13 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     l.n_slices = n_slices;
     l.T = T;
     l.V = V;
-    l.mode = MRF_MATCH_FP32;
+    l.mode = mode;
     l.maps = d_maps;
     l.idx = d_idx;
+    if (mode < MRF_MATCH_FP32 || mode > MRF_MATCH_BF16_TOP32)
+        return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", mode);
+    if (mode != MRF_MATCH_FP32 && T > kLowpMaxT)
+        return mrf_fail(MRF_E_ARG, "low-precision matching supports T <= %d", kLowpMaxT);
     const size_t nblk = (size_t)n_slices * block;
     float *d_seq = nullptr, *d_pblk = nullptr;
     TrCoef *d_coef = nullptr;
@@ -323,6 +327,7 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     double *d_n2 = nullptr;
     int32_t *d_vs = nullptr;
     void *scratch = nullptr;
+    LowpBlockBufs lb{};
     int rc = 0;
     auto alloc = [&](void **p, size_t bytes) {
         if (rc) return;
@@ -337,6 +342,13 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     alloc((void **)&d_pblk, nblk * 4 * sizeof(float));
     if (normalize) alloc((void **)&d_n2, nblk * sizeof(double));
     if (d_b1map) alloc((void **)&d_vs, V * sizeof(int32_t));
+    if (mode != MRF_MATCH_FP32) {
+        const size_t lw = nblk * lowp_ld(nreps) * sizeof(uint32_t);
+        alloc((void **)&lb.hi, lw);
+        if (mode == MRF_MATCH_SPLIT2 || mode == MRF_MATCH_SPLIT3) alloc((void **)&lb.lo, lw);
+        alloc((void **)&lb.best_s, V * sizeof(double));
+        alloc((void **)&lb.best_i, V * sizeof(uint32_t));
+    }
     if (!rc) {
         l.vox_slice = d_vs;
         alloc(&scratch, match_scratch_bytes(l));
@@ -362,10 +374,12 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
             return gen_atoms(echo, d_coef, d_pblk, nb, n_slices, T, nstates, opts, h_real,
                              normalize, d_buf, d_n2, s);
         };
-        rc = launch_match_blocks(l, block, d_buf, scratch, s, gen);
+        rc = launch_match_blocks(l, block, d_buf, mode != MRF_MATCH_FP32 ? &lb : nullptr, scratch, s,
+                                 gen);
     }
     for (void *p : {(void *)d_seq, (void *)d_coef, (void *)d_buf, (void *)d_pblk, (void *)d_n2,
-                    (void *)d_vs, scratch})
+                    (void *)d_vs, scratch, (void *)lb.hi, (void *)lb.lo, (void *)lb.best_s,
+                    (void *)lb.best_i})
         if (p) (void)hipFreeAsync(p, s);
     return rc;
 }

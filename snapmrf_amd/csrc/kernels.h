@@ -66,13 +66,24 @@ struct MatchLaunch {
 // Scratch size in bytes a matching call needs.
 size_t match_scratch_bytes(const MatchLaunch &a);
 int launch_match(const MatchLaunch &a, void *scratch, hipStream_t s);
-// FP32 matching of a dictionary that exists one block at a time: for each
-// block, gen(a0, nb) fills `buf` [n_slices][nb][T] with atoms a0 .. a0+nb-1
-// of every slice (nb <= block), which is then matched; a.atoms is unused.
-// The keys merge across blocks, so the maps equal launch_match's on the
-// whole dictionary.  Scratch as match_scratch_bytes(a).
-int launch_match_blocks(const MatchLaunch &a, size_t block, float2 *buf, void *scratch,
-                        hipStream_t s, const std::function<int(size_t, size_t)> &gen);
+// Matching of a dictionary that exists one block at a time: for each block,
+// gen(a0, nb) fills `buf` [n_slices][nb][T] with atoms a0 .. a0+nb-1 of every
+// slice (nb <= block), which is then matched; a.atoms, a.lowp_hi/lo unused.
+// FP32: the keys merge across blocks, so the maps equal launch_match's on
+// the whole dictionary.  Low-precision modes: each block is converted to
+// BF16 (lb->hi, lb->lo: [n_slices * block][lowp_ld(T)] words), its candidates
+// are rescored against its complex64 rows and merged into a running best
+// per voxel (lb->best_s / best_i, [V]); the candidate set is a superset of
+// the whole-dictionary call's, so the pick scores at least as high.  Scratch
+// as match_scratch_bytes(a).
+struct LowpBlockBufs {
+    uint32_t *hi, *lo;
+    double *best_s;
+    uint32_t *best_i;
+};
+int launch_match_blocks(const MatchLaunch &a, size_t block, float2 *buf,
+                        const LowpBlockBufs *lb, void *scratch, hipStream_t s,
+                        const std::function<int(size_t, size_t)> &gen);
 
 // Row stride (bf16x2 words) of the low-precision copies: T rounded up to a
 // multiple of 32, the pad zero-filled.

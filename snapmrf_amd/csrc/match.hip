@@ -444,8 +444,11 @@ struct LowpArgs {
     const int *ntiles;
     float *part_s;               // [chunks][V][4 * LK] scores (-1 = empty)
     uint32_t *part_i;            // [chunks][V][4 * LK] global atom indices
-    size_t natoms;               // atoms per slice
+    size_t natoms;               // atoms per slice in this launch
     size_t chunk;                // atoms per work-group
+    size_t row_stride;           // dictionary rows per slice in ahi / alo
+    size_t gidx_stride;          // global atom index step per slice
+    size_t gidx_base;            // global index of atom 0 of slice 0
     size_t V;
     int ld;                      // row stride (words)
     int nk;                      // K chunks, ceil(T / KC); nk * KC <= ld
@@ -550,7 +553,8 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
     const size_t a_begin = (size_t)chunk_id * a.chunk;
     if (a_begin >= a.natoms) return;
     const size_t a_end = (a_begin + a.chunk < a.natoms) ? a_begin + a.chunk : a.natoms;
-    const size_t slice_base = (size_t)tile.slice * a.natoms;
+    const size_t row_base = (size_t)tile.slice * a.row_stride;
+    const size_t idx_base = a.gidx_base + (size_t)tile.slice * a.gidx_stride;
     const size_t ldb = (size_t)a.ld * 4;    // row stride in bytes
 
     __shared__ __attribute__((aligned(16))) uint32_t lA[NB * BA];
@@ -587,7 +591,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 
     for (size_t sub = a_begin; sub < a_end; sub += LAT) {
         const size_t nsub = (a_end - sub < (size_t)LAT) ? a_end - sub : (size_t)LAT;
-        const size_t arow0 = (slice_base + sub) * a.ld;
+        const size_t arow0 = (row_base + sub) * a.ld;
         const __amdgpu_buffer_rsrc_t rA = make_rsrc(a.ahi + arow0, nsub * ldb);
         __amdgpu_buffer_rsrc_t rAl = rA;
         if (ASPLIT) rAl = make_rsrc(a.alo + arow0, nsub * ldb);
@@ -718,7 +722,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
         // 16 scores at a time are tested against the list threshold; the
         // (rare) insertions run in a short divergent loop in increasing atom
         // order.
-        const uint32_t lane_atom0 = (uint32_t)(slice_base + sub) + (uint32_t)g;
+        const uint32_t lane_atom0 = (uint32_t)(idx_base + sub) + (uint32_t)g;
         const int nvalid = (int)nsub;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -777,13 +781,44 @@ __device__ __forceinline__ bool cand_better(float s1, uint32_t i1, float s2, uin
 // Per voxel (one wave): merge the chunk candidate lists, keep the top K by
 // low-precision score, rescore them (FP64 accumulation) from the complex64
 // dictionary and keep the best (first index on ties).
-template <int K, int CK>
-__global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
-                                                         const uint32_t *part_i, int nchunks,
-                                                         const float2 *atoms, const float2 *sig,
-                                                         const float *params, size_t V, int T,
-                                                         float *maps, int32_t *idx)
+struct RescoreArgs {
+    const float *part_s;
+    const uint32_t *part_i;
+    int nchunks;
+    const float2 *atoms;        // complex64 rows (the whole dictionary or one block)
+    const float2 *sig;          // [V][T]
+    const float *params;        // [NATOMS][4], global index
+    size_t V;
+    int T;
+    // block layout of `atoms` (nb > 0): global atom gi of slice k = gi / gnat
+    // is row k * nb + (gi - k * gnat - a0); nb = 0: row = gi
+    size_t gnat, a0, nb;
+    // running best over blocks (or null): the best of this call is merged
+    // with it, first index on equal FP64 scores, and the maps are written from it
+    double *best_s;
+    uint32_t *best_i;
+    float *maps;
+    int32_t *idx;
+};
+
+__device__ __forceinline__ size_t atom_row(const RescoreArgs &r, uint32_t gi)
 {
+    if (r.nb == 0) return gi;
+    const size_t k = gi / r.gnat;
+    return k * r.nb + (gi - k * r.gnat - r.a0);
+}
+
+template <int K, int CK>
+__global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
+{
+    const float *part_s = ra.part_s;
+    const uint32_t *part_i = ra.part_i;
+    const int nchunks = ra.nchunks;
+    const float2 *atoms = ra.atoms;
+    const float2 *sig = ra.sig;
+    const float *params = ra.params;
+    const size_t V = ra.V;
+    const int T = ra.T;
     constexpr int PER = kMaxChunks * CK / 64;   // entries per lane
     static_assert(kMaxChunks * CK % 64 == 0, "candidate entries per lane");
     const size_t vox = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -854,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
                 for (int c = 0; c < G; c++)
                     if (c0 + c < nc) {
                         const float4 av = reinterpret_cast<const float4 *>(
-                            atoms + (size_t)cand[c0 + c] * T)[t];
+                            atoms + atom_row(ra, cand[c0 + c]) * T)[t];
                         re[c] += (double)av.x * xv.x + (double)av.y * xv.y +
                                  ((double)av.z * xv.z + (double)av.w * xv.w);
                         im[c] += (double)av.x * xv.y - (double)av.y * xv.x +
@@ -867,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
 #pragma unroll
                 for (int c = 0; c < G; c++)
                     if (c0 + c < nc) {
-                        const float2 av = atoms[(size_t)cand[c0 + c] * T + t];
+                        const float2 av = atoms[atom_row(ra, cand[c0 + c]) * T + t];
                         re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
                         im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
                     }
@@ -891,18 +926,40 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(const float *part_s,
             }
     }
     if (nc == 0) {
-        bs = 0.0;
-        bi = 0u;
+        bs = -1.0;
+        bi = 0xffffffffu;
     }
     if (lane == 0) {
+        if (ra.best_s) {
+            const double ps = ra.best_s[vox];
+            const uint32_t pi = ra.best_i[vox];
+            if (!(bs > ps || (bs == ps && bi < pi))) {
+                bs = ps;
+                bi = pi;
+            }
+            ra.best_s[vox] = bs;
+            ra.best_i[vox] = bi;
+        }
+        if (bs < 0.0) {   // no candidate: the reference leaves index 0
+            bs = 0.0;
+            bi = 0u;
+        }
         const float *p = params + (size_t)bi * 4;
-        maps[vox * 5 + 0] = p[0];
-        maps[vox * 5 + 1] = p[1];
-        maps[vox * 5 + 2] = p[2];
-        maps[vox * 5 + 3] = p[3];
-        maps[vox * 5 + 4] = (float)sqrt(bs);
-        if (idx) idx[vox] = (int32_t)bi;
+        ra.maps[vox * 5 + 0] = p[0];
+        ra.maps[vox * 5 + 1] = p[1];
+        ra.maps[vox * 5 + 2] = p[2];
+        ra.maps[vox * 5 + 3] = p[3];
+        ra.maps[vox * 5 + 4] = (float)sqrt(bs);
+        if (ra.idx) ra.idx[vox] = (int32_t)bi;
     }
+}
+
+__global__ void init_best_kernel(double *best_s, uint32_t *best_i, size_t V)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    best_s[v] = -1.0;
+    best_i[v] = 0xffffffffu;
 }
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -917,7 +974,7 @@ struct Scratch {
     uint32_t *shi, *slo;
     float *part_s;
     uint32_t *part_i;
-    size_t chunk, nchunks;
+    size_t chunk, nchunks, part_chunks;
     size_t bytes;
 };
 
@@ -938,11 +995,19 @@ int lane_k(int mode)
 
 
 
-void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
+// atom chunks per voxel tile: about 2048 work-groups, 8 to kMaxChunks
+size_t lowp_want(size_t max_tiles)
 {
     size_t want = (2048 + max_tiles - 1) / max_tiles;
     if (want < 8) want = 8;
     if (want > (size_t)kMaxChunks) want = kMaxChunks;
+    return want;
+}
+
+// chunking of nat atoms; at most lowp_want(max_tiles) chunks for any nat
+void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
+{
+    const size_t want = lowp_want(max_tiles);
     size_t c = (nat + want - 1) / want;
     if (c < (size_t)kMinChunkAtoms) c = kMinChunkAtoms;
     c = (c + LAT - 1) / LAT * LAT;
@@ -982,11 +1047,13 @@ Scratch carve(void *base, const MatchLaunch &l)
             s.slo = (uint32_t *)p;
             p += align256(l.V * ld * 4);
         }
+        // candidate lists for up to lowp_want chunks (any launch's nat)
+        s.part_chunks = lowp_want(s.max_tiles);
         s.part_s = (float *)p;
         const size_t ck = 4 * (size_t)kAtomHalves * lane_k(l.mode);
-        p += align256(s.nchunks * l.V * ck * 4);
+        p += align256(s.part_chunks * l.V * ck * 4);
         s.part_i = (uint32_t *)p;
-        p += align256(s.nchunks * l.V * ck * 4);
+        p += align256(s.part_chunks * l.V * ck * 4);
     }
     s.bytes = (size_t)(p - (uint8_t *)base);
     return s;
@@ -1078,6 +1145,90 @@ void launch_fp32(const MatchLaunch &l, const Scratch &sc, const float2 *atoms, s
     match_fp32_kernel<<<grid, kFp32Threads, 0, s>>>(a);
 }
 
+// tile-ordered BF16 (and lo) copies of the signals
+int lowp_prepare_signals(const MatchLaunch &l, const Scratch &sc, hipStream_t s)
+{
+    return launch_lowp_prepare(l.sig, sc.perm, l.V, l.T, (int)lowp_ld(l.T), sc.shi,
+                               l.mode == MRF_MATCH_SPLIT3 ? sc.slo : nullptr, s);
+}
+
+RescoreArgs rescore_args(const MatchLaunch &l, const Scratch &sc)
+{
+    RescoreArgs r{};
+    r.part_s = sc.part_s;
+    r.part_i = sc.part_i;
+    r.sig = l.sig;
+    r.params = l.params;
+    r.V = l.V;
+    r.T = l.T;
+    r.maps = l.maps;
+    r.idx = l.idx;
+    return r;
+}
+
+// One low-precision launch (candidates, then their rescore) over atoms
+// [0, nat) of every slice of hi / lo (rows row_stride apart per slice),
+// global index gbase + slice * gstride + atom.
+int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, const uint32_t *lo,
+                size_t nat, size_t row_stride, size_t gstride, size_t gbase, RescoreArgs r,
+                hipStream_t s)
+{
+    size_t chunk = 0, nchunks = 0;
+    lowp_chunks(nat, sc.max_tiles, &chunk, &nchunks);
+    if (nchunks > sc.part_chunks) return mrf_fail(MRF_E_ARG, "candidate storage too small");
+    LowpArgs b{};
+    b.ahi = hi;
+    b.alo = lo;
+    b.shi = sc.shi;
+    b.slo = sc.slo;
+    b.perm = sc.perm;
+    b.tiles = sc.tiles;
+    b.ntiles = sc.ntiles;
+    b.part_s = sc.part_s;
+    b.part_i = sc.part_i;
+    b.natoms = nat;
+    b.chunk = chunk;
+    b.row_stride = row_stride;
+    b.gidx_stride = gstride;
+    b.gidx_base = gbase;
+    b.V = l.V;
+    b.ld = (int)lowp_ld(l.T);
+    b.nch = (int)nchunks;
+    const size_t nwg = (sc.max_tiles * nchunks + 7) / 8 * 8;
+    if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
+    const dim3 grid((unsigned)nwg);
+    r.nchunks = (int)nchunks;
+    // Launch shapes measured on the 446k-atom x 460,800-voxel workload:
+    // eight waves (two per SIMD) were 6-13 % faster than four waves of
+    // 128-atom tiles in every mode; the LDS (two staging buffers + the
+    // lane lists) sets the chunk length and list size per mode.
+    // TOP8/TOP16: 32-timepoint chunks, 3-entry lists.  TOP32: 6-entry lists
+    // need 16-timepoint chunks (3-entry lists at 32 missed the exact pick
+    // on 0.11 % of voxels, 6-entry ones on 0.005 %).  SPLIT2: 16, 4-entry.
+    // SPLIT3: 16, 2-entry.
+    const dim3 rg((unsigned)((l.V + 3) / 4));
+    const dim3 blk(kLowpThreads);
+    if (l.mode == MRF_MATCH_SPLIT3) {
+        b.nk = (l.T + 15) / 16;
+        match_lowp_kernel<3, 16, 2><<<grid, blk, 0, s>>>(b);
+        rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(r);
+    } else if (l.mode == MRF_MATCH_SPLIT2) {
+        b.nk = (l.T + 15) / 16;
+        match_lowp_kernel<2, 16, 4><<<grid, blk, 0, s>>>(b);
+        rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(r);
+    } else if (l.mode == MRF_MATCH_BF16_TOP32) {
+        b.nk = (l.T + 15) / 16;
+        match_lowp_kernel<1, 16, 6><<<grid, blk, 0, s>>>(b);
+        rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(r);
+    } else {
+        b.nk = (l.T + 31) / 32;
+        match_lowp_kernel<1, 32, 3><<<grid, blk, 0, s>>>(b);
+        if (l.mode == MRF_MATCH_BF16_TOP16) rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(r);
+        else rescore_kernel<8, 24><<<rg, kBlock, 0, s>>>(r);
+    }
+    return 0;
+}
+
 }  // namespace
 
 int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
@@ -1088,66 +1239,10 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     Scratch sc = match_setup(l, scratch, s);
     const unsigned vb = (unsigned)((l.V + 255) / 256);
     if (l.mode != MRF_MATCH_FP32) {
-        const int ld = (int)lowp_ld(l.T);
-        if (int rc = launch_lowp_prepare(l.sig, sc.perm, l.V, l.T, ld, sc.shi,
-                                         l.mode == MRF_MATCH_SPLIT3 ? sc.slo : nullptr, s))
-            return rc;
-        LowpArgs b{};
-        b.ahi = l.lowp_hi;
-        b.alo = l.lowp_lo;
-        b.shi = sc.shi;
-        b.slo = sc.slo;
-        b.perm = sc.perm;
-        b.tiles = sc.tiles;
-        b.ntiles = sc.ntiles;
-        b.part_s = sc.part_s;
-        b.part_i = sc.part_i;
-        b.natoms = nat;
-        b.chunk = sc.chunk;
-        b.V = l.V;
-        b.ld = ld;
-        b.nch = (int)sc.nchunks;
-        const size_t nwg = (sc.max_tiles * sc.nchunks + 7) / 8 * 8;
-        if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
-        const dim3 grid((unsigned)nwg);
-        // Launch shapes measured on the 446k-atom x 460,800-voxel workload:
-        // eight waves (two per SIMD) were 6-13 % faster than four waves of
-        // 128-atom tiles in every mode; the LDS (two staging buffers + the
-        // lane lists) sets the chunk length and list size per mode.
-        // TOP8/TOP16: 32-timepoint chunks, 3-entry lists.  TOP32: 6-entry lists
-        // need 16-timepoint chunks (3-entry lists at 32 missed the exact pick
-        // on 0.11 % of voxels, 6-entry ones on 0.005 %).  SPLIT2: 16, 4-entry.
-        // SPLIT3: 16, 2-entry.
-        const int nc = (int)sc.nchunks;
-        const dim3 rg((unsigned)((l.V + 3) / 4));
-        const dim3 blk(kLowpThreads);
-        if (l.mode == MRF_MATCH_SPLIT3) {
-            b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<3, 16, 2><<<grid, blk, 0, s>>>(b);
-            rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
-                                                        l.params, l.V, l.T, l.maps, l.idx);
-        } else if (l.mode == MRF_MATCH_SPLIT2) {
-            b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<2, 16, 4><<<grid, blk, 0, s>>>(b);
-            rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
-                                                        l.params, l.V, l.T, l.maps, l.idx);
-        } else if (l.mode == MRF_MATCH_BF16_TOP32) {
-            b.nk = (l.T + 15) / 16;
-            match_lowp_kernel<1, 16, 6><<<grid, blk, 0, s>>>(b);
-            rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms, l.sig,
-                                                         l.params, l.V, l.T, l.maps, l.idx);
-        } else {
-            b.nk = (l.T + 31) / 32;
-            match_lowp_kernel<1, 32, 3><<<grid, blk, 0, s>>>(b);
-            if (l.mode == MRF_MATCH_BF16_TOP16)
-                rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
-                                                             l.sig, l.params, l.V, l.T, l.maps,
-                                                             l.idx);
-            else
-                rescore_kernel<8, 24><<<rg, kBlock, 0, s>>>(sc.part_s, sc.part_i, nc, l.atoms,
-                                                            l.sig, l.params, l.V, l.T, l.maps,
-                                                            l.idx);
-        }
+        if (int rc = lowp_prepare_signals(l, sc, s)) return rc;
+        RescoreArgs r = rescore_args(l, sc);
+        r.atoms = l.atoms;
+        if (int rc = launch_lowp(l, sc, l.lowp_hi, l.lowp_lo, nat, nat, nat, 0, r, s)) return rc;
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }
@@ -1160,31 +1255,68 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
 }
 
-int launch_match_blocks(const MatchLaunch &l, size_t block, float2 *buf, void *scratch,
-                        hipStream_t s, const std::function<int(size_t, size_t)> &gen)
+int launch_match_blocks(const MatchLaunch &l, size_t block, float2 *buf,
+                        const LowpBlockBufs *lb, void *scratch, hipStream_t s,
+                        const std::function<int(size_t, size_t)> &gen)
 {
-    if (l.mode != MRF_MATCH_FP32)
-        return mrf_fail(MRF_E_UNSUPPORTED, "block-streamed matching is FP32 only");
+    const bool lowp = l.mode != MRF_MATCH_FP32;
     if (block == 0) return mrf_fail(MRF_E_ARG, "block size must be positive");
+    if (lowp && (!lb || !lb->hi || !lb->best_s || !lb->best_i))
+        return mrf_fail(MRF_E_ARG, "block-streamed low-precision matching needs its buffers");
+    const bool need_lo = l.mode == MRF_MATCH_SPLIT2 || l.mode == MRF_MATCH_SPLIT3;
+    if (need_lo && !lb->lo) return mrf_fail(MRF_E_ARG, "split mode needs the lo block buffer");
     if (l.V == 0) return 0;
-    if (int rc = match_check(l)) return rc;
+    MatchLaunch lc = l;
+    if (lowp) lc.lowp_hi = lc.lowp_lo = reinterpret_cast<const uint32_t *>(lb->hi);
+    if (int rc = match_check(lc)) return rc;
     if ((block + kAtomChunk - 1) / kAtomChunk > 65535)
         return mrf_fail(MRF_E_ARG, "block too large");
     Scratch sc = match_setup(l, scratch, s);
+    const unsigned vb = (unsigned)((l.V + 255) / 256);
+    const int ld = (int)lowp_ld(l.T);
+    if (lowp) {
+        if (int rc = lowp_prepare_signals(l, sc, s)) return rc;
+        init_best_kernel<<<vb, 256, 0, s>>>(lb->best_s, lb->best_i, l.V);
+    }
     for (size_t a0 = 0; a0 < l.natoms; a0 += block) {
         const size_t nb = (l.natoms - a0 < block) ? l.natoms - a0 : block;
         if (int rc = gen(a0, nb)) return rc;
+        if (!lowp) {
+            if (l.vox_slice) {
+                launch_fp32(l, sc, buf, nb, nb, l.natoms, a0, s);
+            } else {
+                // every voxel meets every slice: one launch per slice
+                for (int k = 0; k < l.n_slices; k++)
+                    launch_fp32(l, sc, buf + (size_t)k * nb * l.T, nb, 0, 0,
+                                (size_t)k * l.natoms + a0, s);
+            }
+            continue;
+        }
+        // BF16 copies of the block, candidates, and their rescore against
+        // the block's complex64 rows, merged into the running best per voxel
+        const size_t rows = (size_t)l.n_slices * nb;
+        if (int rc = launch_lowp_prepare(buf, nullptr, rows, l.T, ld, lb->hi,
+                                         need_lo ? lb->lo : nullptr, s))
+            return rc;
+        RescoreArgs r = rescore_args(l, sc);
+        r.atoms = buf;
+        r.gnat = l.natoms;
+        r.a0 = a0;
+        r.nb = nb;
+        r.best_s = lb->best_s;
+        r.best_i = lb->best_i;
         if (l.vox_slice) {
-            launch_fp32(l, sc, buf, nb, nb, l.natoms, a0, s);
+            if (int rc = launch_lowp(l, sc, lb->hi, lb->lo, nb, nb, l.natoms, a0, r, s)) return rc;
         } else {
-            // every voxel meets every slice: one launch per slice
-            for (int k = 0; k < l.n_slices; k++)
-                launch_fp32(l, sc, buf + (size_t)k * nb * l.T, nb, 0, 0,
-                            (size_t)k * l.natoms + a0, s);
+            for (int k = 0; k < l.n_slices; k++) {
+                const size_t off = (size_t)k * nb * ld;
+                if (int rc = launch_lowp(l, sc, lb->hi + off, need_lo ? lb->lo + off : nullptr, nb,
+                                         0, 0, (size_t)k * l.natoms + a0, r, s))
+                    return rc;
+            }
         }
     }
-    const unsigned vb = (unsigned)((l.V + 255) / 256);
-    maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
+    if (!lowp) maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
 }

@@ -189,8 +189,8 @@ int main(int argc, char *argv[])
         return EX_USAGE;
     }
 
-    if ((stream || svd_frac > 0.0) && mode != MRF_MATCH_FP32) {
-        std::fprintf(stderr, "mrf: --stream and --svd use the FP32 matcher\n");
+    if (svd_frac > 0.0 && mode != MRF_MATCH_FP32) {
+        std::fprintf(stderr, "mrf: --svd uses the FP32 matcher\n");
         return EX_USAGE;
     }
     if (stream && (svd_frac > 0.0 || out_atoms || !in_img)) {
@@ -312,8 +312,8 @@ int main(int argc, char *argv[])
             if (stream) {
                 std::printf("Constructing and matching the dictionary block by block...\n");
                 if (int rc = mrf_dict_match_ex(nullptr, echo_type, h_mrf.data(), nreps, NREPS,
-                                               d_params, NATOMS, natoms, nstates, nullptr, d_sig,
-                                               nvoxels, d_b1, 0, d_maps, nullptr))
+                                               d_params, NATOMS, natoms, nstates, nullptr, mode,
+                                               d_sig, nvoxels, d_b1, 0, d_maps, nullptr))
                     return fail(rc == MRF_E_ECHO ? EX_USAGE : EX_SOFTWARE, "streamed matching");
             } else if (svd_frac > 0.0) {
                 // per-slice bases when the B1 map picks each voxel's slice

@@ -1,8 +1,9 @@
-"""Dictionary generation streamed through the FP32 matcher
-(mrf_dict_match_ex): the dictionary exists one block of atoms per slice at a
-time, and the maps and indices must equal generating the whole dictionary
-(mrf_dict_ex) and matching it (mrf_match_ex, FP32) bit for bit, for any block
-size, with and without a B1 map.  The whole-dictionary path is pinned to the
+"""Dictionary generation streamed through the matcher (mrf_dict_match_ex): the
+dictionary exists one block of atoms per slice at a time.  FP32: the maps and
+indices must equal generating the whole dictionary (mrf_dict_ex) and matching
+it (mrf_match_ex) bit for bit, for any block size, with and without a B1 map.
+Low-precision modes: the streamed pick must score at least as high in FP64 as
+the whole-dictionary call's.  The whole-dictionary path is pinned to the
 reference's fixtures and the FP64 oracle by test_gpu_epg.py and
 test_gpu_match.py."""
 import numpy as np
@@ -89,3 +90,54 @@ def test_streamed_rejects_partial_dictionaries(gpu):
     # roa_ssfp fills one slice only, so a 3-slice dictionary cannot stream
     with pytest.raises(sm.MRFError):
         sm.dict_match_ex("roa_ssfp", raw, T, d_params, A, natoms, W, sig, 4, d_maps)
+
+
+def _fp64_score(atoms, sig, idx):
+    a = atoms[idx].astype(np.complex128)
+    return np.abs(np.sum(np.conj(a) * sig.astype(np.complex128), axis=1))
+
+
+@pytest.mark.parametrize("use_b1", [True, False])
+def test_streamed_low_precision(gpu, use_b1):
+    """Low-precision modes streamed block by block: each block's candidates
+    are rescored against its own complex64 rows and merged into a running
+    best, a superset of the whole-dictionary call's candidates, so on every
+    voxel the streamed pick's FP64 score is at least the whole call's."""
+    import oracle
+    raw, params, d_params, A, natoms, d_atoms, sig, d_b1, ns = _inputs(gpu, V=2000)
+    b1 = d_b1 if use_b1 else None
+    atoms = d_atoms.cpu().numpy()
+    s_np = sig.cpu().numpy()
+    ld = sm.lowp_ld(T)
+    d_hi = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+    d_lo = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+    sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo)
+    V = s_np.shape[0]
+    for mode in (sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_SPLIT2):
+        d_maps = gpu.empty((V, 5), device="cuda")
+        d_idx = gpu.empty(V, dtype=gpu.int32, device="cuda")
+        sm.match_ex(mode, d_atoms, d_params, T, A, natoms, sig, V, d_maps, d_b1map=b1,
+                    d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo)
+        gpu.cuda.synchronize()
+        whole = d_idx.cpu().numpy()
+        m, i = _streamed_mode(gpu, raw, d_params, A, natoms, sig, b1, 300, mode)
+        assert np.mean(i == whole) >= 0.995, mode
+        assert np.all(_fp64_score(atoms, s_np, i) >= _fp64_score(atoms, s_np, whole) * (1 - 1e-12))
+        assert np.array_equal(m[:, :4], params[i])
+        if use_b1:
+            assert np.all(i // natoms == whole // natoms)     # the B1 map's slice
+    # SPLIT2 streamed: the FP64 argmax wherever the top-2 gap is clear
+    if not use_b1:
+        ridx, rpd, rsec = oracle.match_slice(atoms, s_np)
+        clear = (rpd - rsec) / rpd > 1e-5
+        assert np.array_equal(i[clear], ridx[clear])
+
+
+def _streamed_mode(torch, raw, d_params, A, natoms, sig, d_b1, block, mode):
+    V = sig.shape[0]
+    d_maps = torch.full((V, 5), -7.0, device="cuda")
+    d_idx = torch.full((V,), -7, dtype=torch.int32, device="cuda")
+    sm.dict_match_ex("epg_ssfp", raw, T, d_params, A, natoms, W, sig, V, d_maps, d_b1map=d_b1,
+                     d_idx=d_idx, block_atoms=block, mode=mode)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
