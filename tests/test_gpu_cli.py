@@ -165,3 +165,18 @@ def test_cli_svd(gpu, tmp_path):
     grid_rows = {tuple(r) for r in params.tolist()}
     assert all(tuple(r) in grid_rows for r in m[:, :4].tolist())
     assert np.mean(np.all(m[:, :4] == m32[:, :4], axis=1)) >= 0.95
+
+
+def test_cli_stream_low_precision(gpu, tmp_path):
+    """--stream with --precision bf16_top32: block-streamed BF16 matching;
+    its maps equal the FP32 run's on nearly every voxel."""
+    raw, params, A, natoms, atoms, sig, b1, truth, csv, img, b1f = _setup(gpu, tmp_path)
+    ref, _ = _run(tmp_path, csv, img, b1f)
+    out = str(tmp_path / "s_maps.ra")
+    r = subprocess.run([sm.CLI_PATH, "-t", GRID[0], "-s", GRID[1], "-b", GRID[2], "-r", GRID[3],
+                        "-w", str(W), "-T", str(T), "-M", str(b1f), "-m", out, "--stream",
+                        "--precision", "bf16_top32", str(csv), str(img)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m, m32 = sm.ra_read(out)[0], ref["maps"][0]
+    assert np.mean(np.all(m[:, :4] == m32[:, :4], axis=1)) >= 0.999
