@@ -141,3 +141,53 @@ def _streamed_mode(torch, raw, d_params, A, natoms, sig, d_b1, block, mode):
                      d_idx=d_idx, block_atoms=block, mode=mode)
     torch.cuda.synchronize()
     return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+@pytest.mark.parametrize("T_, V_, block", [(37, 5, 64), (16, 130, 1000), (101, 257, 300)])
+def test_streamed_ragged(gpu, T_, V_, block):
+    """Odd and tiny T, few voxels, blocks larger than a slice or not a
+    multiple of the tile: FP32 streamed equals the whole dictionary; the
+    low-precision modes' streamed picks score at least as high in FP64 as the
+    whole-dictionary call's."""
+    raw = synth.csv_columns(synth.sequence(T_, seed=9), "epg_ssfp")
+    params, A, natoms = sm.build_params("100:100:2000", "20:20:300", "-20:10:20", "0.9:0.1:1.1")
+    d_params = gpu.from_numpy(params).cuda()
+    d_atoms = gpu.empty((A, T_), dtype=gpu.complex64, device="cuda")
+    sm.dict_ex("epg_ssfp", raw, T_, d_params, A, natoms, 8, d_atoms)
+    atoms = d_atoms.cpu().numpy()
+    rng = np.random.default_rng(T_)
+    pick = rng.integers(0, A, V_)
+    s_np = (atoms[pick] * 0.7 + 0.02 * (rng.normal(size=(V_, T_)) + 1j * rng.normal(size=(V_, T_)))
+            ).astype(np.complex64)
+    sig = gpu.from_numpy(s_np).cuda()
+    b1 = gpu.from_numpy(rng.uniform(0.85, 1.15, V_).astype(np.float32)).cuda()
+    for d_b1 in (b1, None):
+        d_maps = gpu.empty((V_, 5), device="cuda")
+        d_idx = gpu.empty(V_, dtype=gpu.int32, device="cuda")
+        sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T_, A, natoms, sig, V_, d_maps,
+                    d_b1map=d_b1, d_idx=d_idx)
+        gpu.cuda.synchronize()
+        i32 = d_idx.cpu().numpy()
+        m, i = _streamed_generic(gpu, raw, T_, d_params, A, natoms, sig, d_b1, block, 0)
+        assert np.array_equal(i, i32) and np.array_equal(m, d_maps.cpu().numpy())
+        ld = sm.lowp_ld(T_)
+        d_hi = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+        d_lo = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+        sm.lowp_prepare(d_atoms, T_, A, d_hi, d_lo)
+        for mode in (sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_SPLIT3):
+            sm.match_ex(mode, d_atoms, d_params, T_, A, natoms, sig, V_, d_maps, d_b1map=d_b1,
+                        d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo)
+            gpu.cuda.synchronize()
+            whole = d_idx.cpu().numpy()
+            m, i = _streamed_generic(gpu, raw, T_, d_params, A, natoms, sig, d_b1, block, mode)
+            assert np.all(_fp64_score(atoms, s_np, i) >= _fp64_score(atoms, s_np, whole) * (1 - 1e-12)), mode
+
+
+def _streamed_generic(torch, raw, T_, d_params, A, natoms, sig, d_b1, block, mode):
+    V = sig.shape[0]
+    d_maps = torch.full((V, 5), -7.0, device="cuda")
+    d_idx = torch.full((V,), -7, dtype=torch.int32, device="cuda")
+    sm.dict_match_ex("epg_ssfp", raw, T_, d_params, A, natoms, 8, sig, V, d_maps, d_b1map=d_b1,
+                     d_idx=d_idx, block_atoms=block, mode=mode)
+    torch.cuda.synchronize()
+    return d_maps.cpu().numpy(), d_idx.cpu().numpy()
