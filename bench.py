@@ -695,7 +695,11 @@ def main() -> int:
                                  "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32,
                                  "traffic": measured_traffic("match"),
                                  "kernel": "match_fp32_kernel",
-                                 "algorithmic_flops_per_launch": mflops}
+                                 "algorithmic_flops_per_launch": mflops,
+                                 # three real products per complex product:
+                                 # the MFMA pipes issue 6 of the 8 flops
+                                 "mfma_flops_per_launch": 0.75 * mflops,
+                                 "frac_mfma_issued": 0.75 * ach_m / PEAK_FP32}
         # the reference interface: MRF_match with a host image and a B1-map
         # file (PCIe upload and file read included, wall clock)
         import tempfile

@@ -1,13 +1,15 @@
 // match.hip -- fused complex-GEMM + argmax matcher for gfx950.
 //
-// S[atom][voxel] = sum_t conj(a_t) s_t is computed as two real GEMMs over the
-// interleaved (re, im) index k:
-//     Re S = sum_k A[k] X[k],   Im S = sum_k A[k] Y[k],
-//     X = (xr0, xi0, xr1, xi1, ...),  Y = (xi0, -xr0, xi1, -xr1, ...)
-// on v_mfma_f32_16x16x4f32 with rows = atoms, columns = voxels.  Each lane
-// reads 4 consecutive k of its row/column with one ds_read_b128 and uses
-// component s in MFMA step s; Y comes from the same 4 floats in registers
-// (x1, -x0, x3, -x2), so the signal is staged once.
+// S[atom][voxel] = sum_t conj(a_t) s_t.  The FP32 kernel forms it from three
+// real GEMMs over t (Gauss's three-multiplication complex product):
+//     P1 = sum ar xr,   P2 = sum ai xi,   P3 = sum (ar - ai)(xr + xi),
+//     Re S = P1 + P2,   Im S = P3 - P1 + P2
+// on v_mfma_f32_16x16x4f32 with rows = atoms, columns = voxels: 6 MFMAs per
+// 8 timepoints where the direct Re/Im split needs 8.  Each lane reads two
+// (re, im) pairs of its row/column with one ds_read_b128; the sums and
+// differences are formed in registers, so atoms and signals are staged once.
+// The cancellation in Im S costs about one extra ulp of the summed
+// magnitudes, far below the score gaps the argmax has to resolve.
 //
 // The score matrix is never written: after each 128-atom sub-tile every lane
 // folds its |S|^2 values into a running (score, index) per voxel, first index
@@ -153,13 +155,16 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
             const size_t ar = sub + st_row + RP * p;
             arow[p] = (ar < a_end) ? a.atoms + (row_base + ar) * (size_t)a.T : nullptr;
         }
-        floatx4 acc_re[NIB][4], acc_im[NIB][4];
+        // three real products (Gauss): P1 = ar.xr, P2 = ai.xi,
+        // P3 = (ar - ai).(xr + xi); Re S = P1 + P2, Im S = P3 - P1 + P2
+        floatx4 acc_p1[NIB][4], acc_p2[NIB][4], acc_p3[NIB][4];
 #pragma unroll
         for (int i = 0; i < NIB; i++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
-                acc_im[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc_p1[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc_p2[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc_p3[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
 
         // global -> registers (zero outside [0, T) and outside the tile); the
@@ -191,27 +196,36 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
             }
 #pragma unroll
             for (int q = 0; q < KF / 16; q++) {
-                floatx4 fa[NIB], fx[4], fy[4];
+                // a lane's 4 floats are (re, im) of two timepoints; MFMA step
+                // u uses timepoint u of every lane group
+                floatx4 fa[NIB], fx[4];
+                float fd[NIB][2], fs[4][2];
 #pragma unroll
-                for (int i = 0; i < NIB; i++)
+                for (int i = 0; i < NIB; i++) {
                     fa[i] = *reinterpret_cast<const floatx4 *>(
                         &lds_a[cb][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
+                    fd[i][0] = fa[i][0] - fa[i][1];
+                    fd[i][1] = fa[i][2] - fa[i][3];
+                }
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     fx[v] = *reinterpret_cast<const floatx4 *>(
                         &lds_b[cb][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
-                    fy[v] = floatx4{fx[v][1], -fx[v][0], fx[v][3], -fx[v][2]};
+                    fs[v][0] = fx[v][0] + fx[v][1];
+                    fs[v][1] = fx[v][2] + fx[v][3];
                 }
 #pragma unroll
-                for (int s = 0; s < 4; s++)
+                for (int u = 0; u < 2; u++)
 #pragma unroll
                     for (int i = 0; i < NIB; i++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            acc_re[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                fa[i][s], fx[v][s], acc_re[i][v], 0, 0, 0);
-                            acc_im[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                fa[i][s], fy[v][s], acc_im[i][v], 0, 0, 0);
+                            acc_p1[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                fa[i][2 * u], fx[v][2 * u], acc_p1[i][v], 0, 0, 0);
+                            acc_p2[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                fa[i][2 * u + 1], fx[v][2 * u + 1], acc_p2[i][v], 0, 0, 0);
+                            acc_p3[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                fd[i][u], fs[v][u], acc_p3[i][v], 0, 0, 0);
                         }
             }
             if (more) {
@@ -232,7 +246,8 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
                 if (atom < a_end) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        const float re = acc_re[i][v][r], im = acc_im[i][v][r];
+                        const float p1 = acc_p1[i][v][r], p2 = acc_p2[i][v][r];
+                        const float re = p1 + p2, im = acc_p3[i][v][r] - p1 + p2;
                         const float sc = re * re + im * im;
                         if (sc > best[v]) {
                             best[v] = sc;
