@@ -169,7 +169,8 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
 
         // global -> registers (zero outside [0, T) and outside the tile); the
         // next chunk's loads are issued before the current chunk's MFMAs and
-        // stored into the other LDS buffer after them (one barrier per chunk)
+        // stored into the other LDS buffer after three quarters of them (one
+        // barrier per chunk)
         float4 ga[NP], gb[NP];
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -183,6 +184,44 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
             *reinterpret_cast<float4 *>(&lds_b[0][(st_row + RP * p) * LS + st_col]) = gb[p];
         }
         __syncthreads();
+        // LDS fragments are double-buffered in registers: the reads for the
+        // next half-chunk go out before the current MFMAs.  Per chunk: MFMAs
+        // of timepoints 0-7 while 8-15 are read; the first half of 8-15;
+        // the next chunk's LDS store and the barrier; its first fragments
+        // are read while the second half runs.  Every read of a buffer is
+        // issued before the barrier that precedes the next store into it.
+        floatx4 fa[2][NIB], fx[2][4];
+        auto read_frags = [&](int b, int q, floatx4 (&ra)[NIB], floatx4 (&rx)[4]) {
+#pragma unroll
+            for (int i = 0; i < NIB; i++)
+                ra[i] = *reinterpret_cast<const floatx4 *>(
+                    &lds_a[b][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                rx[v] = *reinterpret_cast<const floatx4 *>(
+                    &lds_b[b][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
+        };
+        // MFMA step u of a fragment set: timepoint u of every lane group
+        auto mfma_step = [&](const floatx4 (&ra)[NIB], const floatx4 (&rx)[4], int u) {
+            float fd[NIB], fs[4];
+#pragma unroll
+            for (int i = 0; i < NIB; i++) fd[i] = ra[i][2 * u] - ra[i][2 * u + 1];
+#pragma unroll
+            for (int v = 0; v < 4; v++) fs[v] = rx[v][2 * u] + rx[v][2 * u + 1];
+#pragma unroll
+            for (int i = 0; i < NIB; i++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    acc_p1[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        ra[i][2 * u], rx[v][2 * u], acc_p1[i][v], 0, 0, 0);
+                    acc_p2[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        ra[i][2 * u + 1], rx[v][2 * u + 1], acc_p2[i][v], 0, 0, 0);
+                    acc_p3[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        fd[i], fs[v], acc_p3[i][v], 0, 0, 0);
+                }
+        };
+        static_assert(KF == 32, "two 8-timepoint halves per chunk");
+        read_frags(0, 0, fa[0], fx[0]);
         for (int c = 0; c < nchunks; c++) {
             const int cb = c & 1;
             const bool more = c + 1 < nchunks;
@@ -194,40 +233,12 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
                     gb[p] = load_pair(vrow[p], tc, a.T, even_t);
                 }
             }
-#pragma unroll
-            for (int q = 0; q < KF / 16; q++) {
-                // a lane's 4 floats are (re, im) of two timepoints; MFMA step
-                // u uses timepoint u of every lane group
-                floatx4 fa[NIB], fx[4];
-                float fd[NIB][2], fs[4][2];
-#pragma unroll
-                for (int i = 0; i < NIB; i++) {
-                    fa[i] = *reinterpret_cast<const floatx4 *>(
-                        &lds_a[cb][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
-                    fd[i][0] = fa[i][0] - fa[i][1];
-                    fd[i][1] = fa[i][2] - fa[i][3];
-                }
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    fx[v] = *reinterpret_cast<const floatx4 *>(
-                        &lds_b[cb][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
-                    fs[v][0] = fx[v][0] + fx[v][1];
-                    fs[v][1] = fx[v][2] + fx[v][3];
-                }
-#pragma unroll
-                for (int u = 0; u < 2; u++)
-#pragma unroll
-                    for (int i = 0; i < NIB; i++)
-#pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            acc_p1[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                fa[i][2 * u], fx[v][2 * u], acc_p1[i][v], 0, 0, 0);
-                            acc_p2[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                fa[i][2 * u + 1], fx[v][2 * u + 1], acc_p2[i][v], 0, 0, 0);
-                            acc_p3[i][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                fd[i][u], fs[v][u], acc_p3[i][v], 0, 0, 0);
-                        }
-            }
+            read_frags(cb, 1, fa[1], fx[1]);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_step(fa[0], fx[0], 0);
+            mfma_step(fa[0], fx[0], 1);
+            mfma_step(fa[1], fx[1], 0);
+            __builtin_amdgcn_sched_barrier(0);
             if (more) {
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
@@ -236,6 +247,9 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
                 }
             }
             __syncthreads();
+            if (more) read_frags(cb ^ 1, 0, fa[0], fx[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_step(fa[1], fx[1], 1);
         }
         // epilogue: D[row = 4g + r][col = r16] of each 16x16 tile
 #pragma unroll
