@@ -518,7 +518,17 @@ struct RevArgs {
     float wts[4];              // ND = 4
     double *loss;              // ND = 4: [n]
     float *scratch;            // per atom (nck + kCkpt) * 6 (1 + ND) * w floats
+    int seq_lds;               // grad_seq summed per work-group in LDS
 };
+
+// grad_seq partial sums go through LDS up to this T (2T floats)
+constexpr int kSeqLdsMaxT = 8192;
+
+size_t rev_lds_bytes(RevArgs &A)
+{
+    A.seq_lds = A.grad_seq != nullptr && A.T <= kSeqLdsMaxT;
+    return A.seq_lds ? (size_t)2 * A.T * sizeof(float) : 0;
+}
 
 // Fisher matrix from lane 0's running sums; returns false if singular.
 __device__ bool inv4(const double *A, double *X)
@@ -550,11 +560,11 @@ __device__ bool inv4(const double *A, double *X)
     return true;
 }
 
+// One atom's sweeps (the lane group of SUB lanes); gs_lds is the work-group's
+// grad_seq partial sums when A.seq_lds.
 template <int ND, int SUB>
-__global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
+__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds)
 {
-    const size_t a = group_atom<SUB>();
-    if (a >= A.n) return;
     const int k = threadIdx.x & (SUB - 1);
     const int w = A.w, T = A.T;
     const int C = 6 * (1 + ND) * w;
@@ -689,8 +699,13 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
                 ga = group_sum<SUB>(ga);
                 gt = group_sum<SUB>(gt);
                 if (k == 0) {
-                    atomicAdd(&A.grad_seq[t], ga * A.fa_unit);
-                    atomicAdd(&A.grad_seq[T + t], gt);
+                    if (A.seq_lds) {
+                        atomicAdd(&gs_lds[t], ga * A.fa_unit);
+                        atomicAdd(&gs_lds[T + t], gt);
+                    } else {
+                        atomicAdd(&A.grad_seq[t], ga * A.fa_unit);
+                        atomicAdd(&A.grad_seq[T + t], gt);
+                    }
                 }
             }
         }
@@ -711,6 +726,28 @@ __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
         if (k == 0)
 #pragma unroll
             for (int d = 0; d < 4; d++) A.grad_theta[a * 4 + d] = gth[d];
+    }
+}
+
+// grad_seq of the atoms of a work-group is summed in LDS (2T floats of
+// dynamic LDS when A.seq_lds): one LDS add per atom and TR, then one global
+// atomic per entry and work-group instead of one per atom.
+template <int ND, int SUB>
+__global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
+{
+    extern __shared__ float gs_lds[];
+    if (A.seq_lds) {
+        for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) gs_lds[i] = 0.f;
+        __syncthreads();
+    }
+    const size_t a = group_atom<SUB>();
+    if (a < A.n) rev_atom<ND, SUB>(A, a, gs_lds);
+    if (A.seq_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) {
+            const float v = gs_lds[i];
+            if (v != 0.f) atomicAdd(&A.grad_seq[i], v);
+        }
     }
 }
 
@@ -905,8 +942,9 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.grad_seq = grad_seq;
     A.fa_unit = fa_unit;
     A.scratch = scratch;
-    if (w <= 32) rev_kernel<0, 32><<<groups(n, 32), 64 * kWaves, 0, s>>>(A);
-    else rev_kernel<0, 64><<<groups(n, 64), 64 * kWaves, 0, s>>>(A);
+    const size_t lds = rev_lds_bytes(A);
+    if (w <= 32) rev_kernel<0, 32><<<groups(n, 32), 64 * kWaves, lds, s>>>(A);
+    else rev_kernel<0, 64><<<groups(n, 64), 64 * kWaves, lds, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "adjoint launch: %s", hipGetErrorString(e));
 }
@@ -927,8 +965,9 @@ int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const
     for (int d = 0; d < 4; d++) A.wts[d] = wts[d];
     A.loss = loss;
     A.scratch = scratch;
-    if (w <= 32) rev_kernel<4, 32><<<groups(n, 32), 64 * kWaves, 0, s>>>(A);
-    else rev_kernel<4, 64><<<groups(n, 64), 64 * kWaves, 0, s>>>(A);
+    const size_t lds = rev_lds_bytes(A);
+    if (w <= 32) rev_kernel<4, 32><<<groups(n, 32), 64 * kWaves, lds, s>>>(A);
+    else rev_kernel<4, 64><<<groups(n, 64), 64 * kWaves, lds, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "crb launch: %s", hipGetErrorString(e));
 }
