@@ -62,11 +62,13 @@ def test_jacobian_central_differences(gpu):
             assert _rel(J[a, d], fd) < TOL, (a, d)
 
 
-@pytest.mark.parametrize("T,w", [(137, 17), (50, 8), (49, 8), (1000, 30)])
+@pytest.mark.parametrize("T,w", [(137, 17), (50, 8), (49, 8), (1000, 30), (8192, 4), (8193, 4)])
 def test_adjoint_matches_oracle(gpu, T, w):
     """Theta and sequence gradients of L = sum_atoms Re(v^H s), with
     checkpoint boundaries at 50, 100 (T = 137), exactly one segment (50) and
-    a partial single segment (49)."""
+    a partial single segment (49).  The sequence gradients of a work-group's
+    atoms are summed in LDS up to T = 8192 (64 KB) and by global atomics
+    from T = 8193; six atoms leave part of the work-group idle."""
     raw = _raw(T, seed=4)
     n = len(THETAS)
     rng = np.random.default_rng(T)
