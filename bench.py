@@ -61,7 +61,14 @@ def cpu_baseline(seq_raw, params, natoms, T, w, budget_s=8.0):
     oracle.match_slice(a32, sig)
     dt_m = oracle.wall() - t0
     vox_per_s = V / (dt_m * natoms / n2)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=n2 / dt_gen, unit="atoms/s", cores=oracle.num_threads(), kind="port",
+                cpu_model=cpu_model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
                 sample=f"epg_ssfp FP64 oracle, {n2} seeded atoms of the grid x T={T} x w={w}; "
                        f"matching {V} voxels x {n2} atoms scaled to {natoms} atoms/slice",
                 match_value=vox_per_s, match_unit="voxels/s")
