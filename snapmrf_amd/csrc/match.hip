@@ -550,9 +550,11 @@ __device__ __forceinline__ int row_atom(int s)
     return ((s >> 2) & 3) + 4 * (s & 3) + (s & ~15);
 }
 
-// KC timepoints per K chunk (16 or 32), two LDS buffers; LK candidates per
+// KC timepoints per K chunk (16 or 32), NB LDS staging buffers (2: one
+// barrier per chunk; 1: a second barrier once the fragments are read, which
+// leaves LDS for longer lane lists); LK candidates per
 // lane and voxel.
-template <int NPROD, int KC, int LK>
+template <int NPROD, int KC, int LK, int NB>
 __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 {
     constexpr int AH = kAtomHalves;
@@ -566,8 +568,8 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
     constexpr int PA = LAT / RPP;
     constexpr int PX = LVT / RPP;
     constexpr int BA = LAT * LSB, BX = LVT * LSB;
-    constexpr int NB = 2;
     static_assert(KC == 16 || KC == 32, "K chunk");
+    static_assert(NB == 1 || NB == 2, "staging buffers");
     static_assert(4 * (NB * (BA * (ASPLIT ? 2 : 1) + BX * (SSPLIT ? 2 : 1)) +
                        2 * AH * 4 * LVT * LK) <= 160 * 1024,
                   "LDS budget");
@@ -730,11 +732,17 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
         stage_store(0);
         __syncthreads();
         for (int c = 0; c < nk; c++) {
-            read_frags(c & 1);
+            read_frags(NB == 2 ? (c & 1) : 0);
             __builtin_amdgcn_sched_barrier(0);
+            if constexpr (NB == 1) {
+                // one staging buffer: once every wave holds its fragments
+                // the buffer takes the next chunk
+                __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+                __syncthreads();
+            }
             stage_load(c + 1 < nk ? c + 1 : c);
             mfmas();
-            stage_store((c + 1) & 1);
+            stage_store(NB == 2 ? ((c + 1) & 1) : 0);
 #pragma unroll
             for (int j = 0; j < NLD; j++) {
                 __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // VMEM read
@@ -1229,29 +1237,31 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     r.nchunks = (int)nchunks;
     // Launch shapes measured on the 446k-atom x 460,800-voxel workload:
     // eight waves (two per SIMD) were 6-13 % faster than four waves of
-    // 128-atom tiles in every mode; the LDS (two staging buffers + the
-    // lane lists) sets the chunk length and list size per mode.
-    // TOP8/TOP16: 32-timepoint chunks, 3-entry lists.  TOP32: 6-entry lists
-    // need 16-timepoint chunks (3-entry lists at 32 missed the exact pick
-    // on 0.11 % of voxels, 6-entry ones on 0.005 %).  SPLIT2: 16, 4-entry.
-    // SPLIT3: 16, 2-entry.
+    // 128-atom tiles in every mode; the LDS (staging buffers + the lane
+    // lists) sets the chunk length and list size per mode.
+    // TOP8/TOP16: 32-timepoint chunks, 3-entry lists, two buffers.  TOP32:
+    // 6-entry lists (3-entry ones missed the exact pick on 0.11 % of voxels,
+    // 6-entry ones on 0.005 %) fit 32-timepoint chunks with one staging
+    // buffer (2 % faster than two buffers of 16).  SPLIT2: 16, 4-entry.
+    // SPLIT3: 16, 2-entry (their fragments of 32-timepoint chunks do not fit
+    // the registers of two waves per SIMD).
     const dim3 rg((unsigned)((l.V + 3) / 4));
     const dim3 blk(kLowpThreads);
     if (l.mode == MRF_MATCH_SPLIT3) {
         b.nk = (l.T + 15) / 16;
-        match_lowp_kernel<3, 16, 2><<<grid, blk, 0, s>>>(b);
+        match_lowp_kernel<3, 16, 2, 2><<<grid, blk, 0, s>>>(b);
         rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(r);
     } else if (l.mode == MRF_MATCH_SPLIT2) {
         b.nk = (l.T + 15) / 16;
-        match_lowp_kernel<2, 16, 4><<<grid, blk, 0, s>>>(b);
+        match_lowp_kernel<2, 16, 4, 2><<<grid, blk, 0, s>>>(b);
         rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(r);
     } else if (l.mode == MRF_MATCH_BF16_TOP32) {
-        b.nk = (l.T + 15) / 16;
-        match_lowp_kernel<1, 16, 6><<<grid, blk, 0, s>>>(b);
+        b.nk = (l.T + 31) / 32;
+        match_lowp_kernel<1, 32, 6, 1><<<grid, blk, 0, s>>>(b);
         rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(r);
     } else {
         b.nk = (l.T + 31) / 32;
-        match_lowp_kernel<1, 32, 3><<<grid, blk, 0, s>>>(b);
+        match_lowp_kernel<1, 32, 3, 2><<<grid, blk, 0, s>>>(b);
         if (l.mode == MRF_MATCH_BF16_TOP16) rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(r);
         else rescore_kernel<8, 24><<<rg, kBlock, 0, s>>>(r);
     }
