@@ -906,7 +906,10 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
     // double, so the candidates are ranked by their (nearly) exact scores
     // and equal scores resolve to the lowest index.  Candidates go in groups
     // of at most 16 (the signal row is re-read from cache per group), which
-    // keeps the accumulators within 64 registers.
+    // keeps the accumulators within 64 registers.  Every slot of a group
+    // loads a valid row (slots past the nc candidates re-read candidate 0's
+    // row and are ignored), so the loop body has no branches and a group's
+    // row loads are all in flight together.
     constexpr int G = K < 16 ? K : 16;
     double bs = -1.0;
     uint32_t bi = 0u;
@@ -915,34 +918,39 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
     for (int c0 = 0; c0 < K; c0 += G) {
         if (c0 >= nc) break;
         double re[G], im[G];
+        const float2 *row[G];
 #pragma unroll
-        for (int c = 0; c < G; c++) re[c] = im[c] = 0.0;
+        for (int c = 0; c < G; c++) {
+            re[c] = im[c] = 0.0;
+            row[c] = atoms + atom_row(ra, c0 + c < nc ? cand[c0 + c] : cand[0]) * T;
+        }
         if ((T & 1) == 0) {
             // even T: rows are 16-byte aligned, two timepoints per load
             const float4 *x4 = reinterpret_cast<const float4 *>(x);
             for (int t = lane; t < T / 2; t += 64) {
                 const float4 xv = x4[t];
+                float4 av[G];
 #pragma unroll
-                for (int c = 0; c < G; c++)
-                    if (c0 + c < nc) {
-                        const float4 av = reinterpret_cast<const float4 *>(
-                            atoms + atom_row(ra, cand[c0 + c]) * T)[t];
-                        re[c] += (double)av.x * xv.x + (double)av.y * xv.y +
-                                 ((double)av.z * xv.z + (double)av.w * xv.w);
-                        im[c] += (double)av.x * xv.y - (double)av.y * xv.x +
-                                 ((double)av.z * xv.w - (double)av.w * xv.z);
-                    }
+                for (int c = 0; c < G; c++) av[c] = reinterpret_cast<const float4 *>(row[c])[t];
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    re[c] += (double)av[c].x * xv.x + (double)av[c].y * xv.y +
+                             ((double)av[c].z * xv.z + (double)av[c].w * xv.w);
+                    im[c] += (double)av[c].x * xv.y - (double)av[c].y * xv.x +
+                             ((double)av[c].z * xv.w - (double)av[c].w * xv.z);
+                }
             }
         } else {
             for (int t = lane; t < T; t += 64) {
                 const float2 xv = x[t];
+                float2 av[G];
 #pragma unroll
-                for (int c = 0; c < G; c++)
-                    if (c0 + c < nc) {
-                        const float2 av = atoms[atom_row(ra, cand[c0 + c]) * T + t];
-                        re[c] += (double)av.x * xv.x + (double)av.y * xv.y;
-                        im[c] += (double)av.x * xv.y - (double)av.y * xv.x;
-                    }
+                for (int c = 0; c < G; c++) av[c] = row[c][t];
+#pragma unroll
+                for (int c = 0; c < G; c++) {
+                    re[c] += (double)av[c].x * xv.x + (double)av[c].y * xv.y;
+                    im[c] += (double)av[c].x * xv.y - (double)av[c].y * xv.x;
+                }
             }
         }
 #pragma unroll
