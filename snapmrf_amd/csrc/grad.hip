@@ -156,6 +156,77 @@ __device__ __forceinline__ TrK tr_coef(const DTr &q, const Atom &p)
     return c;
 }
 
+// Atom-dependent per-TR coefficients, cached SUB TRs at a time: lane k of a
+// group computes TR b * SUB + k (three sincosf, four expf) into LDS and the
+// group reads each TR's entry back, so a lane does 1 / SUB of that work.  The
+// two groups of a wave walk the same TRs, so a block changes for the whole
+// wave at once; a group's entries are written and read by that wave only.
+struct alignas(16) TrA {
+    float sa, ca, e1a, e2a, e1b, e2b;
+    float2 pa, pb;
+    float pad[2];
+};
+static_assert(sizeof(TrA) == 48, "TrA layout");
+
+constexpr int kCoefLds = kWaves * 64;   // TrA entries per work-group
+
+template <int SUB>
+struct CoefCache {
+    TrA *blk;   // the group's SUB entries
+    int cur;    // block held, -1: none
+};
+
+template <int SUB>
+__device__ __forceinline__ CoefCache<SUB> coef_cache(TrA *lds)
+{
+    return {lds + (threadIdx.x >> 6) * 64 + ((threadIdx.x & 63) / SUB) * SUB, -1};
+}
+
+template <int SUB>
+__device__ __forceinline__ TrK coef(CoefCache<SUB> &cc, const DTr *seq, int T, const Atom &p,
+                                    int t, int k)
+{
+    const int b = t / SUB;
+    if (b != cc.cur) {
+        const int tt = b * SUB + k;
+        if (tt < T) {
+            const TrK c = tr_coef(seq[tt], p);
+            TrA e;
+            e.sa = c.sa;
+            e.ca = c.ca;
+            e.e1a = c.e1a;
+            e.e2a = c.e2a;
+            e.e1b = c.e1b;
+            e.e2b = c.e2b;
+            e.pa = c.pa;
+            e.pb = c.pb;
+            e.pad[0] = e.pad[1] = 0.f;
+            cc.blk[k] = e;
+        }
+        cc.cur = b;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const TrA e = cc.blk[t - b * SUB];
+    const DTr q = seq[t];
+    TrK c;
+    c.al = q.al;
+    c.sa = e.sa;
+    c.ca = e.ca;
+    c.zz = {q.cph, q.sph};
+    c.z2 = {q.c2p, q.s2p};
+    c.te = q.te;
+    c.t2 = q.t2;
+    c.e1a = e.e1a;
+    c.e2a = e.e2a;
+    c.e1b = e.e1b;
+    c.e2b = e.e2b;
+    c.pa = e.pa;
+    c.pb = e.pb;
+    return c;
+}
+
 // --- forward operators on the augmented state (lane k) -------------------
 
 template <int ND>
@@ -487,10 +558,12 @@ __global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T,
     if (a >= n) return;
     const int k = threadIdx.x & (SUB - 1);
     const Atom p = load_atom(theta, a);
+    __shared__ TrA coef_lds[kCoefLds];
+    CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
     Aug<4> y;
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
-        const TrK c = tr_coef(seq[t], p);
+        const TrK c = coef<SUB>(cc, seq, T, p, t, k);
         Aug<4> r;
         f_tr<4, SUB>(y, c, p, k, w, &r);
         if (k == 0) {
@@ -563,7 +636,7 @@ __device__ bool inv4(const double *A, double *X)
 // One atom's sweeps (the lane group of SUB lanes); gs_lds is the work-group's
 // grad_seq partial sums when A.seq_lds.
 template <int ND, int SUB>
-__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds)
+__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds, TrA *coef_lds)
 {
     const int k = threadIdx.x & (SUB - 1);
     const int w = A.w, T = A.T;
@@ -571,6 +644,7 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     float *ck = A.scratch + a * (size_t)(A.nck + kCkpt) * C;
     float *seg = ck + (size_t)A.nck * C;
     const Atom p = load_atom(A.theta, a);
+    CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
 
     // forward sweep: checkpoints every kCkpt TRs; Fisher sums (ND = 4)
     Aug<ND> y;
@@ -580,7 +654,7 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     for (int i = 0; i < 10; i++) F[i] = 0.f;
     for (int t = 0; t < T; t++) {
         if (t % kCkpt == 0) st_store(ck + (size_t)(t / kCkpt) * C, w, k, y);
-        const TrK c = tr_coef(A.seq[t], p);
+        const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
         if constexpr (ND > 0) {
             Aug<ND> r;
             f_tr<ND, SUB>(y, c, p, k, w, &r);
@@ -642,12 +716,12 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
         st_load(ck + (size_t)sgi * C, w, k, y);
         for (int t = t0; t < t1; t++) {
             st_store(seg + (size_t)(t - t0) * C, w, k, y);
-            f_tr<ND, SUB>(y, tr_coef(A.seq[t], p), p, k, w, nullptr);
+            f_tr<ND, SUB>(y, coef<SUB>(cc, A.seq, T, p, t, k), p, k, w, nullptr);
         }
         for (int t = t1 - 1; t >= t0; t--) {
             Aug<ND> yn;
             st_load(seg + (size_t)(t - t0) * C, w, k, yn);
-            const TrK c = tr_coef(A.seq[t], p);
+            const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
             float ga = 0.f, gt = 0.f;
             // the TR's forward chain once: x1 = rf(yn), x2 = relax_a(x1),
             // x3 = phase_a(x2); the reverse steps take their inputs from it
@@ -736,12 +810,13 @@ template <int ND, int SUB>
 __global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
 {
     extern __shared__ float gs_lds[];
+    __shared__ TrA coef_lds[kCoefLds];
     if (A.seq_lds) {
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) gs_lds[i] = 0.f;
         __syncthreads();
     }
     const size_t a = group_atom<SUB>();
-    if (a < A.n) rev_atom<ND, SUB>(A, a, gs_lds);
+    if (a < A.n) rev_atom<ND, SUB>(A, a, gs_lds, coef_lds);
     if (A.seq_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) {
@@ -774,14 +849,16 @@ struct GnSums {
 };
 
 template <int ND, int SUB>
-__device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const float2 *x, int k)
+__device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const float2 *x, int k,
+                         TrA *coef_lds)
 {
     GnSums g{};
+    CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);   // p differs per pass
     Aug<ND> y;
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
         Aug<ND> r;
-        f_tr<ND, SUB>(y, tr_coef(seq[t], p), p, k, w, &r);
+        f_tr<ND, SUB>(y, coef<SUB>(cc, seq, T, p, t, k), p, k, w, &r);
         if (k == 0) {
             const float2 s = r.fp[0], xv = x[t];
             g.S += re_dot(s, s);
@@ -825,6 +902,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
 {
     const size_t v = group_atom<SUB>();
     if (v >= A.V) return;
+    __shared__ TrA coef_lds[kCoefLds];
     const int k = threadIdx.x & (SUB - 1);
     const float2 *x = A.sig + v * A.T;
     float xx = 0.f;
@@ -833,7 +911,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
-    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
+    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
     float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
     c = bcast0<SUB>(c);
     if (k == 0 && A.cost) A.cost[v * (A.iters + 1)] = c;
@@ -872,7 +950,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
         for (int d = 0; d < 4; d++) trial[d] = bcast0<SUB>(trial[d]);
         float ct = INFINITY;
         if (theta_ok(trial)) {
-            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k);
+            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k, coef_lds);
             ct = xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
             ct = bcast0<SUB>(ct);
         }
@@ -881,8 +959,8 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
             for (int d = 0; d < 4; d++) th[d] = trial[d];
             c = ct;
             lam = fmax(lam * 0.1, 1e-9);
-            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
-            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k);
+            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
+            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
         } else {
             lam = fmin(lam * 10.0, 1e12);
         }

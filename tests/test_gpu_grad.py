@@ -160,7 +160,11 @@ def test_gauss_newton_recovers_off_grid_truths(gpu):
     th, pd, cost = d_th.cpu().numpy(), d_pd.cpu().numpy(), d_cost.cpu().numpy()
     assert np.all(np.diff(cost, axis=1) <= 0.0)
     xx = np.sum(np.abs(x) ** 2, axis=1)
-    assert np.all(cost[:, -1] <= 1e-6 * xx)
+    # the converged cost is |x|^2 - |s^H x|^2 / |s|^2 of two FP32 sums over T
+    # terms of size |x|^2: its floor is rounding, about sqrt(T) eps |x|^2
+    # (= 1.2e-6 |x|^2 here, and converged costs come out negative as often as
+    # positive), so the bound is 1e-5 |x|^2, four orders below the start
+    assert np.all(cost[:, -1] <= 1e-5 * xx)
     assert np.all(cost[:, 0] > 100 * cost[:, -1])
     err = np.abs(th - truth)
     assert np.all(err[:, 0] <= 1e-3 * truth[:, 0])
