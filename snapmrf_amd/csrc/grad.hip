@@ -156,6 +156,14 @@ __device__ __forceinline__ TrK tr_coef(const DTr &q, const Atom &p)
     return c;
 }
 
+// orders LDS writes and reads of one wave's lanes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Atom-dependent per-TR coefficients, cached SUB TRs at a time: lane k of a
 // group computes TR b * SUB + k (three sincosf, four expf) into LDS and the
 // group reads each TR's entry back, so a lane does 1 / SUB of that work.  The
@@ -204,9 +212,7 @@ __device__ __forceinline__ TrK coef(CoefCache<SUB> &cc, const DTr *seq, int T, c
             cc.blk[k] = e;
         }
         cc.cur = b;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
     }
     const TrA e = cc.blk[t - b * SUB];
     const DTr q = seq[t];
@@ -225,6 +231,20 @@ __device__ __forceinline__ TrK coef(CoefCache<SUB> &cc, const DTr *seq, int T, c
     c.pa = e.pa;
     c.pb = e.pb;
     return c;
+}
+
+// Readouts (state 0 and its ND tangents, on lane 0) staged for SUB TRs in
+// LDS, so that the group's lanes consume them one TR per lane once per block
+// (coalesced stores, sums spread over the lanes) instead of lane 0 alone
+// every TR.  kReadLds float2 per work-group (ND <= 4).
+constexpr int kReadLds = kWaves * 64 * 5;
+
+// the group's region holds SUB x 5 float2 whatever ND is, so passes of
+// different ND in different waves never overlap
+template <int SUB>
+__device__ __forceinline__ float2 *read_stage(float2 *lds)
+{
+    return lds + ((threadIdx.x >> 6) * 64 + ((threadIdx.x & 63) / SUB) * SUB) * 5;
 }
 
 // --- forward operators on the augmented state (lane k) -------------------
@@ -559,18 +579,30 @@ __global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T,
     const int k = threadIdx.x & (SUB - 1);
     const Atom p = load_atom(theta, a);
     __shared__ TrA coef_lds[kCoefLds];
+    __shared__ float2 read_lds[kReadLds];
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
+    float2 *rd = read_stage<SUB>(read_lds);
     Aug<4> y;
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
         const TrK c = coef<SUB>(cc, seq, T, p, t, k);
         Aug<4> r;
         f_tr<4, SUB>(y, c, p, k, w, &r);
-        if (k == 0) {
-            if (sig) sig[a * T + t] = r.fp[0];
-            if (jac)
+        const int j = t % SUB;
+        if (k == 0)
 #pragma unroll
-                for (int d = 0; d < 4; d++) jac[(a * 4 + d) * T + t] = r.fp[1 + d];
+            for (int d = 0; d < 5; d++) rd[j * 5 + d] = r.fp[d];
+        if (j == SUB - 1 || t == T - 1) {
+            // lane k stores TR t - j + k: coalesced rows of s and J
+            wave_sync();
+            if (k <= j) {
+                const size_t tt = (size_t)(t - j + k);
+                if (sig) sig[a * T + tt] = rd[k * 5];
+                if (jac)
+#pragma unroll
+                    for (int d = 0; d < 4; d++) jac[(a * 4 + d) * T + tt] = rd[k * 5 + 1 + d];
+            }
+            wave_sync();
         }
     }
 }
@@ -850,31 +882,54 @@ struct GnSums {
 
 template <int ND, int SUB>
 __device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const float2 *x, int k,
-                         TrA *coef_lds)
+                         TrA *coef_lds, float2 *read_lds)
 {
     GnSums g{};
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);   // p differs per pass
+    float2 *rd = read_stage<SUB>(read_lds);
     Aug<ND> y;
     f_init(y, p, k);
     for (int t = 0; t < T; t++) {
         Aug<ND> r;
         f_tr<ND, SUB>(y, coef<SUB>(cc, seq, T, p, t, k), p, k, w, &r);
-        if (k == 0) {
-            const float2 s = r.fp[0], xv = x[t];
-            g.S += re_dot(s, s);
-            g.Sx = c_add(g.Sx, c_mul(c_conj(s), xv));
-            if constexpr (ND > 0) {
-                int i = 0;
+        const int j = t % SUB;
+        if (k == 0)
 #pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    const float2 ds = r.fp[1 + d];
-                    g.U[d] = c_add(g.U[d], c_mul(c_conj(s), ds));
-                    g.W[d] = c_add(g.W[d], c_mul(c_conj(ds), xv));
+            for (int d = 0; d <= ND; d++) rd[j * (1 + ND) + d] = r.fp[d];
+        if (j == SUB - 1 || t == T - 1) {
+            // lane k folds TR t - j + k into its partial sums
+            wave_sync();
+            if (k <= j) {
+                const float2 *e = rd + k * (1 + ND);
+                const float2 s = e[0], xv = x[t - j + k];
+                g.S += re_dot(s, s);
+                g.Sx = c_add(g.Sx, c_mul(c_conj(s), xv));
+                if constexpr (ND > 0) {
+                    int i = 0;
 #pragma unroll
-                    for (int e = d; e < 4; e++, i++) g.D[i] = c_add(g.D[i], c_mul(c_conj(ds), r.fp[1 + e]));
+                    for (int d = 0; d < 4; d++) {
+                        const float2 ds = e[1 + d];
+                        g.U[d] = c_add(g.U[d], c_mul(c_conj(s), ds));
+                        g.W[d] = c_add(g.W[d], c_mul(c_conj(ds), xv));
+#pragma unroll
+                        for (int q = d; q < 4; q++, i++) g.D[i] = c_add(g.D[i], c_mul(c_conj(ds), e[1 + q]));
+                    }
                 }
             }
+            wave_sync();
         }
+    }
+    // the group's partial sums
+    g.S = group_sum<SUB>(g.S);
+    g.Sx = {group_sum<SUB>(g.Sx.x), group_sum<SUB>(g.Sx.y)};
+    if constexpr (ND > 0) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            g.U[d] = {group_sum<SUB>(g.U[d].x), group_sum<SUB>(g.U[d].y)};
+            g.W[d] = {group_sum<SUB>(g.W[d].x), group_sum<SUB>(g.W[d].y)};
+        }
+#pragma unroll
+        for (int i = 0; i < 10; i++) g.D[i] = {group_sum<SUB>(g.D[i].x), group_sum<SUB>(g.D[i].y)};
     }
     return g;
 }
@@ -903,6 +958,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
     const size_t v = group_atom<SUB>();
     if (v >= A.V) return;
     __shared__ TrA coef_lds[kCoefLds];
+    __shared__ float2 read_lds[kReadLds];
     const int k = threadIdx.x & (SUB - 1);
     const float2 *x = A.sig + v * A.T;
     float xx = 0.f;
@@ -911,7 +967,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
-    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
+    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
     float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
     c = bcast0<SUB>(c);
     if (k == 0 && A.cost) A.cost[v * (A.iters + 1)] = c;
@@ -950,7 +1006,7 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
         for (int d = 0; d < 4; d++) trial[d] = bcast0<SUB>(trial[d]);
         float ct = INFINITY;
         if (theta_ok(trial)) {
-            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k, coef_lds);
+            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k, coef_lds, read_lds);
             ct = xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
             ct = bcast0<SUB>(ct);
         }
@@ -959,8 +1015,8 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
             for (int d = 0; d < 4; d++) th[d] = trial[d];
             c = ct;
             lam = fmax(lam * 0.1, 1e-9);
-            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
-            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds);
+            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
+            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
         } else {
             lam = fmin(lam * 10.0, 1e12);
         }
