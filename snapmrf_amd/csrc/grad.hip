@@ -838,8 +838,9 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
 // grad_seq of the atoms of a work-group is summed in LDS (2T floats of
 // dynamic LDS when A.seq_lds): one LDS add per atom and TR, then one global
 // atomic per entry and work-group instead of one per atom.
+// ND = 0 fits 128 VGPRs: four waves per SIMD (6 % faster than three)
 template <int ND, int SUB>
-__global__ __launch_bounds__(64 * kWaves) void rev_kernel(RevArgs A)
+__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 4 : 1) void rev_kernel(RevArgs A)
 {
     extern __shared__ float gs_lds[];
     __shared__ TrA coef_lds[kCoefLds];
