@@ -259,7 +259,10 @@ int mrf_epg_jacobian(void *stream, const float *h_mrf, size_t nreps, size_t NREP
  * [n][nreps]: d_grad_theta [n][4] = dL/dtheta (optional) and d_grad_seq
  * [2][nreps] += (dL/dFA_raw, dL/dTR_raw) summed over atoms (optional,
  * accumulated: zero it first).  States are checkpointed every 50 TRs and
- * recomputed per segment. */
+ * recomputed per segment.  Scratch: about 4 * (ceil(nreps / 50) + 50) * 6 *
+ * nstates bytes per atom (5x that for mrf_crb_loss_grad), allocated per call
+ * on the stream; calls needing more than 4 GiB (or MRF_DIFF_SCRATCH_BYTES when
+ * set in the environment) run the atoms in equal batches. */
 int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
                     const float *d_theta, size_t n, int nstates, const mrf_complex *d_cot,
                     float *d_grad_theta, float *d_grad_seq);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -482,7 +483,33 @@ void build_dtr(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<DTr> 
 }
 
 constexpr float kFaUnit = (float)(60.0 * M_PI / 180.0);
-constexpr size_t kDiffScratchBudget = (size_t)1 << 30;   // bytes per launch batch
+constexpr size_t kDiffScratchBudget = (size_t)4 << 30;   // bytes per launch batch
+
+// scratch bytes per launch batch: MRF_DIFF_SCRATCH_BYTES when set (read at
+// every call), else 4 GiB
+size_t diff_budget()
+{
+    if (const char *e = std::getenv("MRF_DIFF_SCRATCH_BYTES")) {
+        char *end = nullptr;
+        const unsigned long long v = std::strtoull(e, &end, 10);
+        if (end != e && v > 0) return (size_t)v;
+    }
+    return kDiffScratchBudget;
+}
+
+// atoms per launch batch: at most the scratch budget allows, the batches of
+// one call equal in size (multiples of 8 atoms, a work-group's worth) so the
+// last launch is not a short tail
+size_t diff_batch(size_t n, size_t per_atom)
+{
+    size_t cap = diff_budget() / per_atom;
+    if (cap < 1) cap = 1;
+    if (cap >= n) return n;
+    const size_t nb = (n + cap - 1) / cap;
+    size_t b = (n + nb - 1) / nb;
+    b = (b + 7) / 8 * 8;
+    return b <= cap ? b : cap;
+}
 
 int diff_check(const float *h_mrf, size_t nreps, size_t NREPS, int nstates)
 {
@@ -535,9 +562,7 @@ extern "C" int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, s
     hipStream_t s = (hipStream_t)stream;
     const int T = (int)nreps;
     const size_t per = diff_scratch_floats(0, T, nstates) * sizeof(float);
-    size_t batch = kDiffScratchBudget / per;
-    if (batch < 1) batch = 1;
-    if (batch > n) batch = n;
+    const size_t batch = diff_batch(n, per);
     DTr *d_seq = nullptr;
     if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
     float *scratch = nullptr;
@@ -587,9 +612,7 @@ extern "C" int mrf_crb_loss_grad(void *stream, const float *h_mrf, size_t nreps,
     hipStream_t s = (hipStream_t)stream;
     const int T = (int)nreps;
     const size_t per = diff_scratch_floats(4, T, nstates) * sizeof(float);
-    size_t batch = kDiffScratchBudget / per;
-    if (batch < 1) batch = 1;
-    if (batch > n) batch = n;
+    const size_t batch = diff_batch(n, per);
     DTr *d_seq = nullptr;
     if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
     float *scratch = nullptr;

@@ -257,3 +257,36 @@ def test_bad_arguments(gpu):
         sm.epg_jacobian(raw, d_th, 1, 65, d_sig=gpu.empty((1, 10), dtype=gpu.complex64, device="cuda"))
     with pytest.raises(RuntimeError, match="iters"):
         sm.gn_refine(raw, gpu.empty((1, 10), dtype=gpu.complex64, device="cuda"), 1, 8, -1, d_th)
+
+
+def test_adjoint_and_crb_batches(gpu, monkeypatch):
+    """A scratch budget of two atoms' worth (MRF_DIFF_SCRATCH_BYTES) splits a
+    call into batches: theta gradients and CRB losses equal one launch bit for
+    bit, sequence gradients to FP32 summation order."""
+    T, w = 137, 17
+    raw = _raw(T, seed=4)
+    n = len(THETAS)
+    rng = np.random.default_rng(21)
+    v = (rng.normal(size=(n, T)) + 1j * rng.normal(size=(n, T))).astype(np.complex64)
+    d_th = gpu.from_numpy(THETAS).cuda()
+    d_v = gpu.from_numpy(v).cuda()
+    wts = np.array([1.0, 1.0, 0.01, 0.1], np.float32)
+
+    def run():
+        g_th = gpu.empty((n, 4), device="cuda")
+        g_seq = gpu.zeros((2, T), device="cuda")
+        sm.epg_adjoint(raw, d_th, n, w, d_v, d_grad_theta=g_th, d_grad_seq=g_seq)
+        loss = gpu.empty(n, dtype=gpu.float64, device="cuda")
+        c_seq = gpu.zeros((2, T), device="cuda")
+        sm.crb_loss_grad(raw, d_th, n, w, wts, loss, d_grad_seq=c_seq)
+        gpu.cuda.synchronize()
+        return [a.cpu().numpy() for a in (g_th, g_seq, loss, c_seq)]
+
+    one = run()
+    per = 4 * ((T + 49) // 50 + 50) * 6 * w          # adjoint scratch bytes per atom
+    monkeypatch.setenv("MRF_DIFF_SCRATCH_BYTES", str(2 * per))
+    many = run()
+    assert np.array_equal(one[0], many[0])
+    assert np.array_equal(one[2], many[2])
+    for a, b in ((one[1], many[1]), (one[3], many[3])):
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
