@@ -93,7 +93,7 @@ def test_config1_fp32_vs_fp64_oracle(gpu):
     _check_against_oracle(maps, idx, atoms, params, sig)
 
 
-@pytest.mark.parametrize("T,V", [(37, 1), (37, 129), (16, 300), (1000, 257)])
+@pytest.mark.parametrize("T,V", [(1, 7), (3, 64), (15, 33), (37, 1), (37, 129), (16, 300), (1000, 257)])
 def test_ragged_shapes(gpu, T, V):
     rng = np.random.default_rng(T * 1000 + V)
     A = 300
@@ -300,7 +300,7 @@ def test_split_bf16_reference_kat(gpu, golden):
     assert np.all(fp64_score(atoms, sig, got16) >= 0.999)
 
 
-@pytest.mark.parametrize("T,V,A", [(37, 5, 300), (16, 130, 1000), (1000, 257, 700)])
+@pytest.mark.parametrize("T,V,A", [(3, 7, 300), (15, 33, 500), (37, 5, 300), (16, 130, 1000), (1000, 257, 700)])
 def test_lowp_ragged(gpu, T, V, A):
     rng = np.random.default_rng(T + V)
     atoms = (rng.normal(size=(A, T)) + 1j * rng.normal(size=(A, T))).astype(np.complex64)
