@@ -595,8 +595,14 @@ extern "C" int mrf_gn_refine(void *stream, const float *h_mrf, size_t nreps, siz
     hipStream_t s = (hipStream_t)stream;
     DTr *d_seq = nullptr;
     if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    void *scratch = nullptr;
+    if (hipMallocAsync(&scratch, gn_scratch_bytes(V), s) != hipSuccess) {
+        (void)hipFreeAsync(d_seq, s);
+        return mrf_fail(MRF_E_NOMEM, "Gauss-Newton state (%zu bytes)", gn_scratch_bytes(V));
+    }
     int rc = launch_gn(d_seq, (int)nreps, nstates, reinterpret_cast<const float2 *>(d_sig), V,
-                       d_theta, iters, d_pd, d_cost, s);
+                       d_theta, iters, d_pd, d_cost, scratch, s);
+    (void)hipFreeAsync(scratch, s);
     (void)hipFreeAsync(d_seq, s);
     return rc;
 }

@@ -864,6 +864,7 @@ __global__ __launch_bounds__(64 * kWaves, ND == 0 ? 4 : 1) void rev_kernel(RevAr
 // per voxel with the complex proton density eliminated (variable
 // projection): cost c = |x|^2 - |s^H x|^2 / |s|^2.  A step is kept only if
 // it lowers c, so the cost history is monotone.
+struct GnState;
 struct GnArgs {
     const DTr *seq;
     int T, w, iters;
@@ -872,9 +873,12 @@ struct GnArgs {
     float *theta;              // [V][4] in/out
     float *pd;                 // [V] or null: |s^H x| / |s|
     float *cost;               // [V][iters + 1] or null
+    GnState *st;               // [V] state between launches
+    int *list;                 // [V] voxels due a tangent pass
+    int *count;                // their number
 };
 
-// lane 0's sums of one pass: S = s^H s, Sx = s^H x, U_d = s^H ds_d,
+// a group's sums of one pass: S = s^H s, Sx = s^H x, U_d = s^H ds_d,
 // W_d = ds_d^H x, D_de = ds_d^H ds_e (d <= e)
 struct GnSums {
     float S;
@@ -953,8 +957,21 @@ __device__ __forceinline__ Atom make_atom(const float *th)
     return p;
 }
 
+// Gauss-Newton state of one voxel between the launches of an iteration.
+struct GnState {
+    float th[4];       // current estimate
+    float trial[4];    // this iteration's step
+    float c, ct;       // cost at th and at trial
+    float xx;          // |x|^2
+    float S, Sx[2];    // s^H s and s^H x at th (the proton density)
+    float hS, hSx[2];  // the same at trial
+    double lam;        // Levenberg-Marquardt damping
+    GnSums g;          // tangent-pass sums at th
+};
+
+// group's voxel: one group of SUB lanes per voxel (or per list entry)
 template <int SUB>
-__global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
+__global__ __launch_bounds__(64 * kWaves) void gn_init_kernel(GnArgs A)
 {
     const size_t v = group_atom<SUB>();
     if (v >= A.V) return;
@@ -968,66 +985,137 @@ __global__ __launch_bounds__(64 * kWaves) void gn_kernel(GnArgs A)
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
-    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
-    float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
-    c = bcast0<SUB>(c);
-    if (k == 0 && A.cost) A.cost[v * (A.iters + 1)] = c;
-    double lam = 1e-3;
-    for (int it = 0; it < A.iters; it++) {
-        float trial[4] = {0.f, 0.f, 0.f, 0.f};
-        if (k == 0) {
-            // normal equations in FP64
-            const double S = g.S;
-            const double rr = g.Sx.x / S, ri = g.Sx.y / S;     // rho = Sx / S
-            const double r2 = rr * rr + ri * ri;
-            double Am[16], b[4];
-            int i = 0;
-            for (int d = 0; d < 4; d++)
-                for (int e = d; e < 4; e++, i++) {
-                    // Re[D_de - conj(U_d) U_e / S]
-                    const double cu = (double)g.U[d].x * g.U[e].x + (double)g.U[d].y * g.U[e].y;
-                    Am[4 * d + e] = Am[4 * e + d] = r2 * ((double)g.D[i].x - cu / S);
-                }
-            for (int d = 0; d < 4; d++) {
-                // g_d = -Re[conj(rho) (W_d - rho conj(U_d))]
-                const double wr = g.W[d].x - (rr * g.U[d].x + ri * g.U[d].y);
-                const double wi = g.W[d].y - (ri * g.U[d].x - rr * g.U[d].y);
-                b[d] = rr * wr + ri * wi;   // = -g_d
-            }
-            double Md[16], Mi[16];
-            for (int q = 0; q < 16; q++) Md[q] = Am[q];
-            for (int d = 0; d < 4; d++) Md[5 * d] = Am[5 * d] * (1.0 + lam);
-            double dl[4] = {0, 0, 0, 0};
-            if (inv4(Md, Mi))
-                for (int d = 0; d < 4; d++)
-                    for (int e = 0; e < 4; e++) dl[d] += Mi[4 * d + e] * b[e];
-            for (int d = 0; d < 4; d++) trial[d] = (float)(th[d] + dl[d]);
-        }
+    const GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
+    if (k == 0) {
+        GnState &st = A.st[v];
+        const float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
 #pragma unroll
-        for (int d = 0; d < 4; d++) trial[d] = bcast0<SUB>(trial[d]);
-        float ct = INFINITY;
-        if (theta_ok(trial)) {
-            const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), x, k, coef_lds, read_lds);
-            ct = xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
-            ct = bcast0<SUB>(ct);
+        for (int d = 0; d < 4; d++) st.th[d] = th[d];
+        st.c = c;
+        st.xx = xx;
+        st.S = g.S;
+        st.Sx[0] = g.Sx.x;
+        st.Sx[1] = g.Sx.y;
+        st.lam = 1e-3;
+        st.g = g;
+        if (A.cost) A.cost[v * (A.iters + 1)] = c;
+    }
+}
+
+// the damped normal equations in FP64 (one thread per voxel); thread 0 also
+// clears the accepted-voxel count of this iteration
+__global__ void gn_solve_kernel(GnArgs A)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) *A.count = 0;
+    if (v >= A.V) return;
+    GnState &st = A.st[v];
+    const GnSums &g = st.g;
+    const double S = g.S;
+    const double rr = g.Sx.x / S, ri = g.Sx.y / S;     // rho = Sx / S
+    const double r2 = rr * rr + ri * ri;
+    double Am[16], b[4];
+    int i = 0;
+    for (int d = 0; d < 4; d++)
+        for (int e = d; e < 4; e++, i++) {
+            // Re[D_de - conj(U_d) U_e / S]
+            const double cu = (double)g.U[d].x * g.U[e].x + (double)g.U[d].y * g.U[e].y;
+            Am[4 * d + e] = Am[4 * e + d] = r2 * ((double)g.D[i].x - cu / S);
         }
-        if (ct < c) {
+    for (int d = 0; d < 4; d++) {
+        // g_d = -Re[conj(rho) (W_d - rho conj(U_d))]
+        const double wr = g.W[d].x - (rr * g.U[d].x + ri * g.U[d].y);
+        const double wi = g.W[d].y - (ri * g.U[d].x - rr * g.U[d].y);
+        b[d] = rr * wr + ri * wi;   // = -g_d
+    }
+    double Md[16], Mi[16];
+    for (int q = 0; q < 16; q++) Md[q] = Am[q];
+    for (int d = 0; d < 4; d++) Md[5 * d] = Am[5 * d] * (1.0 + st.lam);
+    double dl[4] = {0, 0, 0, 0};
+    if (inv4(Md, Mi))
+        for (int d = 0; d < 4; d++)
+            for (int e = 0; e < 4; e++) dl[d] += Mi[4 * d + e] * b[e];
+    for (int d = 0; d < 4; d++) st.trial[d] = (float)(st.th[d] + dl[d]);
+}
+
+// the cost at the trial step: a primal pass (no tangents, so this kernel
+// keeps the register budget of the primal state)
+template <int SUB>
+__global__ __launch_bounds__(64 * kWaves) void gn_eval_kernel(GnArgs A)
+{
+    const size_t v = group_atom<SUB>();
+    if (v >= A.V) return;
+    __shared__ TrA coef_lds[kCoefLds];
+    __shared__ float2 read_lds[kReadLds];
+    const int k = threadIdx.x & (SUB - 1);
+    GnState &st = A.st[v];
+    float trial[4];
 #pragma unroll
-            for (int d = 0; d < 4; d++) th[d] = trial[d];
-            c = ct;
-            lam = fmax(lam * 0.1, 1e-9);
-            if (it + 1 < A.iters) g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
-            else g = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
-        } else {
-            lam = fmin(lam * 10.0, 1e12);
-        }
-        if (k == 0 && A.cost) A.cost[v * (A.iters + 1) + it + 1] = c;
+    for (int d = 0; d < 4; d++) trial[d] = st.trial[d];
+    float ct = INFINITY, hS = 0.f, hx = 0.f, hy = 0.f;
+    if (theta_ok(trial)) {
+        const GnSums h = gn_pass<0, SUB>(A.seq, A.T, A.w, make_atom(trial), A.sig + v * A.T, k,
+                                         coef_lds, read_lds);
+        ct = st.xx - (h.Sx.x * h.Sx.x + h.Sx.y * h.Sx.y) / h.S;
+        hS = h.S;
+        hx = h.Sx.x;
+        hy = h.Sx.y;
     }
     if (k == 0) {
-#pragma unroll
-        for (int d = 0; d < 4; d++) A.theta[v * 4 + d] = th[d];
-        if (A.pd) A.pd[v] = sqrtf(g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / sqrtf(g.S);
+        st.ct = ct;
+        st.hS = hS;
+        st.hSx[0] = hx;
+        st.hSx[1] = hy;
     }
+}
+
+// keep the step if it lowers the cost (one thread per voxel); voxels that
+// moved and have iterations left are listed for a new tangent pass
+__global__ void gn_accept_kernel(GnArgs A, int it)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= A.V) return;
+    GnState &st = A.st[v];
+    if (st.ct < st.c) {
+        for (int d = 0; d < 4; d++) st.th[d] = st.trial[d];
+        st.c = st.ct;
+        st.S = st.hS;
+        st.Sx[0] = st.hSx[0];
+        st.Sx[1] = st.hSx[1];
+        st.lam = fmax(st.lam * 0.1, 1e-9);
+        if (it + 1 < A.iters) A.list[atomicAdd(A.count, 1)] = (int)v;
+    } else {
+        st.lam = fmin(st.lam * 10.0, 1e12);
+    }
+    if (A.cost) A.cost[v * (A.iters + 1) + it + 1] = st.c;
+}
+
+// tangent pass at the new estimate of the listed voxels
+template <int SUB>
+__global__ __launch_bounds__(64 * kWaves) void gn_tangent_kernel(GnArgs A)
+{
+    const size_t p = group_atom<SUB>();
+    if (p >= (size_t)*A.count) return;
+    __shared__ TrA coef_lds[kCoefLds];
+    __shared__ float2 read_lds[kReadLds];
+    const int k = threadIdx.x & (SUB - 1);
+    const size_t v = (size_t)A.list[p];
+    GnState &st = A.st[v];
+    float th[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) th[d] = st.th[d];
+    const GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), A.sig + v * A.T, k, coef_lds,
+                                     read_lds);
+    if (k == 0) st.g = g;
+}
+
+__global__ void gn_final_kernel(GnArgs A)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= A.V) return;
+    const GnState &st = A.st[v];
+    for (int d = 0; d < 4; d++) A.theta[v * 4 + d] = st.th[d];
+    if (A.pd) A.pd[v] = sqrtf(st.Sx[0] * st.Sx[0] + st.Sx[1] * st.Sx[1]) / sqrtf(st.S);
 }
 
 }  // namespace
@@ -1107,8 +1195,17 @@ int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "crb launch: %s", hipGetErrorString(e));
 }
 
+size_t gn_scratch_bytes(size_t V)
+{
+    return (V * sizeof(GnState) + 255) / 256 * 256 + (V + 1) * sizeof(int);
+}
+
+// Each iteration: FP64 solve per voxel, primal pass at the trial, accept,
+// tangent pass for the voxels that moved (compacted, so a rejected voxel
+// never waits in a wave for an accepted one).  The decisions are those of a
+// single-kernel loop; the passes get their own register budgets.
 int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
-              float *pd, float *cost, hipStream_t s)
+              float *pd, float *cost, void *scratch, hipStream_t s)
 {
     if (V == 0) return 0;
     GnArgs A{};
@@ -1121,8 +1218,26 @@ int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *
     A.theta = theta;
     A.pd = pd;
     A.cost = cost;
-    if (w <= 32) gn_kernel<32><<<groups(V, 32), 64 * kWaves, 0, s>>>(A);
-    else gn_kernel<64><<<groups(V, 64), 64 * kWaves, 0, s>>>(A);
+    A.st = reinterpret_cast<GnState *>(scratch);
+    A.list = reinterpret_cast<int *>(reinterpret_cast<char *>(scratch) +
+                                     (V * sizeof(GnState) + 255) / 256 * 256);
+    A.count = A.list + V;
+    const int sub = w <= 32 ? 32 : 64;
+    const dim3 gg = groups(V, sub), gb(64 * kWaves);
+    const dim3 vb((unsigned)((V + 255) / 256));
+    if (sub == 32) gn_init_kernel<32><<<gg, gb, 0, s>>>(A);
+    else gn_init_kernel<64><<<gg, gb, 0, s>>>(A);
+    for (int it = 0; it < iters; it++) {
+        gn_solve_kernel<<<vb, 256, 0, s>>>(A);
+        if (sub == 32) gn_eval_kernel<32><<<gg, gb, 0, s>>>(A);
+        else gn_eval_kernel<64><<<gg, gb, 0, s>>>(A);
+        gn_accept_kernel<<<vb, 256, 0, s>>>(A, it);
+        if (it + 1 < iters) {
+            if (sub == 32) gn_tangent_kernel<32><<<gg, gb, 0, s>>>(A);
+            else gn_tangent_kernel<64><<<gg, gb, 0, s>>>(A);
+        }
+    }
+    gn_final_kernel<<<vb, 256, 0, s>>>(A);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "gn launch: %s", hipGetErrorString(e));
 }

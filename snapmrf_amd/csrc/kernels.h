@@ -113,8 +113,10 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
                    float *grad_theta, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
 int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const float *wts,
                double *loss, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
+// scratch bytes of launch_gn for V voxels
+size_t gn_scratch_bytes(size_t V);
 int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
-              float *pd, float *cost, hipStream_t s);
+              float *pd, float *cost, void *scratch, hipStream_t s);
 
 // B1 slice per voxel: argmin_k |b1map[v] - slice_b1[k]| (strict <, init 10).
 int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,
