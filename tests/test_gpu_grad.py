@@ -290,3 +290,30 @@ def test_adjoint_and_crb_batches(gpu, monkeypatch):
     assert np.array_equal(one[2], many[2])
     for a, b in ((one[1], many[1]), (one[3], many[3])):
         assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
+
+
+def test_gauss_newton_zero_iterations(gpu):
+    """iters = 0: theta is returned unchanged, the proton density and the
+    cost are those of the start, |s^H x| / |s| and |x|^2 - |s^H x|^2 / |s|^2
+    with s the GPU's own unnormalised signal there."""
+    T, w, V = 200, 16, 24
+    raw = _raw(T, seed=6)
+    rng = np.random.default_rng(8)
+    truth = np.stack([rng.uniform(300, 2500, V), rng.uniform(30, 300, V),
+                      rng.uniform(-30, 30, V), rng.uniform(0.85, 1.15, V)], 1).astype(np.float32)
+    s_true, _ = _jac(gpu, raw, truth, w)
+    x = (s_true * 0.7).astype(np.complex64)
+    start = (truth * np.float32(1.05)).astype(np.float32)
+    s0, _ = _jac(gpu, raw, start, w)
+    d_th = gpu.from_numpy(start.copy()).cuda()
+    d_pd = gpu.empty(V, device="cuda")
+    d_cost = gpu.empty((V, 1), device="cuda")
+    sm.gn_refine(raw, gpu.from_numpy(x).cuda(), V, w, 0, d_th, d_pd=d_pd, d_cost=d_cost)
+    gpu.cuda.synchronize()
+    assert np.array_equal(d_th.cpu().numpy(), start)
+    s64, x64 = s0.astype(np.complex128), x.astype(np.complex128)
+    sx = np.abs(np.sum(np.conj(s64) * x64, axis=1))
+    ss = np.sum(np.abs(s64) ** 2, axis=1)
+    xx = np.sum(np.abs(x64) ** 2, axis=1)
+    assert np.allclose(d_pd.cpu().numpy(), sx / np.sqrt(ss), rtol=1e-5)
+    assert np.all(np.abs(d_cost.cpu().numpy()[:, 0] - (xx - sx ** 2 / ss)) <= 1e-5 * xx)
