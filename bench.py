@@ -165,6 +165,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                           ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
                           ("bf16_top16", sm.MRF_MATCH_BF16_TOP16, steps),
                           ("bf16_top32", sm.MRF_MATCH_BF16_TOP32, steps),
+                          ("bf16_top64", sm.MRF_MATCH_BF16_TOP64, steps),
                           ("split2", sm.MRF_MATCH_SPLIT2, steps),
                           ("split3", sm.MRF_MATCH_SPLIT3, steps)):
         d_maps = torch.empty((V, 5), device="cuda")
@@ -183,7 +184,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                      "truth_agree": float(np.mean(out[name][1] == truth))}
         log(f"precision form {name}: {ms:.1f} ms/call")
     i32 = out["fp32"][1]
-    lowp_names = ("bf16_top8", "bf16_top16", "bf16_top32", "split2", "split3")
+    lowp_names = ("bf16_top8", "bf16_top16", "bf16_top32", "bf16_top64", "split2", "split3")
     for name in lowp_names:
         res[name]["argmax_agree_fp32"] = float(np.mean(out[name][1] == i32))
         res[name]["speedup_vs_fp32"] = res[name]["voxels_per_s"] / res["fp32"]["voxels_per_s"]
@@ -448,6 +449,65 @@ GRID_C1 = ("100:20:2000", "20:10:300", "0", "1")
 FLOP_PER_STATE_FISP = 78
 
 
+def precision_check(sm, torch, np, d_atoms, d_params, params, T, A, natoms, d_sig, V, d_b1,
+                    fp32_ms, fp32_idx, fp32_maps, stream, timed, oracle_voxels):
+    """BF16_TOP32, BF16_TOP64 and SPLIT2 on a configuration's own dictionary and signals:
+    time, ratio to the FP32 matcher, argmax agreement with it, and (on a
+    seeded voxel subset) map errors against the FP64 oracle argmax over the
+    same complex64 dictionary, with the FP32 matcher's beside them."""
+    ld = sm.lowp_ld(T)
+    d_hi = torch.empty((A, ld), dtype=torch.int32, device="cuda")
+    d_lo = torch.empty((A, ld), dtype=torch.int32, device="cuda")
+    sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo, stream=stream)
+    res, maps = {}, {"fp32": fp32_maps}
+    for name, mode in (("bf16_top32", sm.MRF_MATCH_BF16_TOP32), ("bf16_top64", sm.MRF_MATCH_BF16_TOP64),
+                       ("split2", sm.MRF_MATCH_SPLIT2)):
+        d_maps = torch.empty((V, 5), device="cuda")
+        d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+
+        def run():
+            sm.match_ex(mode, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps, d_b1map=d_b1,
+                        d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo, stream=stream)
+        run()
+        ms = timed(run, 2)
+        idx = d_idx.cpu().numpy()
+        maps[name] = d_maps.cpu().numpy()
+        res[name] = {"ms_per_call": ms, "speedup_vs_fp32": fp32_ms / ms,
+                     "argmax_agree_fp32": float(np.mean(idx == fp32_idx))}
+    del d_hi, d_lo
+    if oracle_voxels:
+        import oracle
+        sub = np.sort(np.random.default_rng(41).choice(V, min(V, oracle_voxels), replace=False))
+        sig = d_sig[torch.from_numpy(sub).cuda()].cpu().numpy()
+        slices = A // natoms
+        if d_b1 is not None:
+            slice_b1 = params[::natoms, 3]
+            ks = np.array([oracle.b1_slice(float(b), slice_b1) for b in d_b1.cpu().numpy()[sub]])
+        else:
+            ks = np.zeros(len(sub), np.int64)
+        ridx = np.zeros(len(sub), np.int64)
+        t0 = time.time()
+        for k in np.unique(ks):
+            sel = ks == k
+            if d_b1 is None and slices > 1:
+                rows = d_atoms.cpu().numpy()
+            else:
+                rows = d_atoms[k * natoms:(k + 1) * natoms].cpu().numpy()
+            i, _, _ = oracle.match_slice(rows, sig[sel])
+            ridx[sel] = i + (0 if d_b1 is None else k * natoms)
+        ref = params[ridx]
+        for name, m in maps.items():
+            mm = m[sub]
+            e = {"map_exact": float(np.mean(np.all(mm[:, :4] == ref, axis=1))),
+                 "T1_mae_ms": float(np.mean(np.abs(mm[:, 0] - ref[:, 0]))),
+                 "T2_mae_ms": float(np.mean(np.abs(mm[:, 1] - ref[:, 1]))),
+                 "B0_mae_hz": float(np.mean(np.abs(mm[:, 2] - ref[:, 2])))}
+            res.setdefault(name, {})["fp64_oracle"] = e
+        res["fp64_oracle_voxels"] = int(len(sub))
+        log(f"precision check: FP64 oracle on {len(sub)} voxels, {time.time() - t0:.1f} s")
+    return res
+
+
 def config_fisp(sm, torch, np, synth, T, stream, timed, cpu):
     """The CPU-runnable configuration: EPG-FISP, T1 100:20:2000, T2 20:10:300,
     w = 20, T = 1000; 64 x 64 voxels (PD ~ U(0.5, 1), SNR 30 on the atom RMS)
@@ -479,12 +539,14 @@ def config_fisp(sm, torch, np, synth, T, stream, timed, cpu):
     mrun()
     m_ms = timed(mrun, 5)
     idx = d_idx.cpu().numpy()
+    prec = precision_check(sm, torch, np, d_atoms, d_params, params, T, A, natoms, d_sig, V, None,
+                           m_ms, idx, d_maps.cpu().numpy(), stream, timed, V if cpu else 0)
     res = {"workload": f"epg_fisp {A} atoms (T1 {GRID_C1[0]}, T2 {GRID_C1[1]}) x T={T} x w={w}, "
                        f"FP32 matching of 64x64 voxels", "atoms": A, "voxels": V,
            "gen_ms": gen_ms, "gen_atoms_per_s": A / (gen_ms * 1e-3),
            "gen_tflops_algorithmic": A * T * w * FLOP_PER_STATE_FISP / (gen_ms * 1e-3) / 1e12,
            "match_ms": m_ms, "match_voxels_per_s": V / (m_ms * 1e-3),
-           "match_truth_agree": float(np.mean(idx == truth))}
+           "match_truth_agree": float(np.mean(idx == truth)), "precision": prec}
     if cpu:
         import oracle
         seq = oracle.seq_prepare("epg_fisp", raw)
@@ -707,6 +769,12 @@ def main() -> int:
                                  # the MFMA pipes issue 6 of the 8 flops
                                  "mfma_flops_per_launch": 0.75 * mflops,
                                  "frac_mfma_issued": 0.75 * ach_m / PEAK_FP32}
+        if not args.skip_lowp:
+            out["precision_c2"] = precision_check(
+                sm, torch, np, d_atoms, d_params, params, T, A, natoms, d_sig, V, d_b1, match_ms,
+                idx, d_maps.cpu().numpy(), stream, timed,
+                0 if (args.no_cpu or args.oracle_voxels == 0) else 256)
+            log(f"precision on this workload: {json.dumps(out['precision_c2'])[:200]}")
         # the reference interface: MRF_match with a host image and a B1-map
         # file (PCIe upload and file read included, wall clock)
         import tempfile

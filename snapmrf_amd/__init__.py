@@ -23,6 +23,7 @@ MRF_MATCH_SPLIT3 = 2
 MRF_MATCH_BF16_TOP16 = 3
 MRF_MATCH_SPLIT2 = 4
 MRF_MATCH_BF16_TOP32 = 5
+MRF_MATCH_BF16_TOP64 = 6
 
 ECHO_TYPES = ("epg_ssfp", "roa_ssfp", "epg_sr", "epg_se", "epg_tse1", "epg_tse2", "epg_fisp")
 

@@ -317,7 +317,7 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     l.mode = mode;
     l.maps = d_maps;
     l.idx = d_idx;
-    if (mode < MRF_MATCH_FP32 || mode > MRF_MATCH_BF16_TOP32)
+    if (mode < MRF_MATCH_FP32 || mode > MRF_MATCH_BF16_TOP64)
         return mrf_fail(MRF_E_UNSUPPORTED, "matching mode %d not available", mode);
     if (mode != MRF_MATCH_FP32 && T > kLowpMaxT)
         return mrf_fail(MRF_E_ARG, "low-precision matching supports T <= %d", kLowpMaxT);

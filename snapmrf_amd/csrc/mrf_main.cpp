@@ -47,7 +47,7 @@ void print_usage()
     std::fprintf(stderr, "\t-T, --timepoints n\t\t number of time points\n");
     std::fprintf(stderr, "\t-G, --gridsize n\t\t set GPU gridsize (accepted, unused)\n");
     std::fprintf(stderr, "\t-B, --blocksize n\t\t set GPU blocksize (accepted, unused)\n");
-    std::fprintf(stderr, "\t    --precision fp32|bf16|bf16_top16|bf16_top32|split2|split3\t matcher precision (default fp32)\n");
+    std::fprintf(stderr, "\t    --precision fp32|bf16|bf16_top16|bf16_top32|bf16_top64|split2|split3\t matcher precision (default fp32)\n");
     std::fprintf(stderr, "\t    --stream\t\t\t generate and match block by block (no -a)\n");
     std::fprintf(stderr, "\t    --svd FRAC\t\t\t match in the rank-K SVD space keeping energy fraction FRAC\n");
     std::fprintf(stderr, "\t-h\t\t\t\t show this help\n");
@@ -182,6 +182,7 @@ int main(int argc, char *argv[])
     if (!std::strcmp(precision, "bf16")) mode = MRF_MATCH_BF16_TOP8;
     else if (!std::strcmp(precision, "bf16_top16")) mode = MRF_MATCH_BF16_TOP16;
     else if (!std::strcmp(precision, "bf16_top32")) mode = MRF_MATCH_BF16_TOP32;
+    else if (!std::strcmp(precision, "bf16_top64")) mode = MRF_MATCH_BF16_TOP64;
     else if (!std::strcmp(precision, "split2")) mode = MRF_MATCH_SPLIT2;
     else if (!std::strcmp(precision, "split3")) mode = MRF_MATCH_SPLIT3;
     else if (std::strcmp(precision, "fp32")) {

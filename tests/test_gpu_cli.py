@@ -82,7 +82,7 @@ def test_cli_end_to_end(gpu, tmp_path):
     assert np.mean(idx == truth) > 0.95
 
 
-@pytest.mark.parametrize("precision", ["split2", "bf16_top16", "bf16_top32"])
+@pytest.mark.parametrize("precision", ["split2", "bf16_top16", "bf16_top32", "bf16_top64"])
 def test_cli_low_precision(gpu, tmp_path, precision):
     raw, params, A, natoms, atoms, sig, b1, truth, csv, img, b1f = _setup(gpu, tmp_path)
     res32, _ = _run(tmp_path, csv, img, b1f)

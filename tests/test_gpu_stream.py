@@ -113,7 +113,8 @@ def test_streamed_low_precision(gpu, use_b1):
     d_lo = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
     sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo)
     V = s_np.shape[0]
-    for mode in (sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_SPLIT2):
+    for mode in (sm.MRF_MATCH_BF16_TOP16, sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_BF16_TOP64,
+                 sm.MRF_MATCH_SPLIT2):
         d_maps = gpu.empty((V, 5), device="cuda")
         d_idx = gpu.empty(V, dtype=gpu.int32, device="cuda")
         sm.match_ex(mode, d_atoms, d_params, T, A, natoms, sig, V, d_maps, d_b1map=b1,
@@ -174,7 +175,7 @@ def test_streamed_ragged(gpu, T_, V_, block):
         d_hi = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
         d_lo = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
         sm.lowp_prepare(d_atoms, T_, A, d_hi, d_lo)
-        for mode in (sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_SPLIT3):
+        for mode in (sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_BF16_TOP64, sm.MRF_MATCH_SPLIT3):
             sm.match_ex(mode, d_atoms, d_params, T_, A, natoms, sig, V_, d_maps, d_b1map=d_b1,
                         d_idx=d_idx, d_lowp=d_hi, d_lowp_lo=d_lo)
             gpu.cuda.synchronize()
