@@ -876,7 +876,12 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
             id[q] = part_i[o];
         }
     }
-    uint32_t cand[K];
+    // the selected candidates are wave-uniform; above 32 of them they live
+    // in LDS (written by lane 0, read back as broadcasts) rather than in K
+    // registers per lane, which would spill
+    constexpr bool kCandLds = K > 32;
+    __shared__ uint32_t cand_lds[kBlock / 64][kCandLds ? K : 1];
+    uint32_t cand[kCandLds ? 1 : K];
     int nc = 0;
 #pragma unroll
     for (int r = 0; r < K; r++) {
@@ -896,11 +901,20 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
                 bi = oi;
             }
         }
-        cand[r] = bi;
+        if constexpr (kCandLds) {
+            if (lane == 0) cand_lds[threadIdx.x >> 6][r] = bi;
+        } else {
+            cand[r] = bi;
+        }
         if (bs >= 0.0f) nc = r + 1;
 #pragma unroll
         for (int q = 0; q < PER; q++)
             if (id[q] == bi) s[q] = -1.0f;
+    }
+    if constexpr (kCandLds) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // Rescore with FP64 accumulation: products of two floats are exact in
     // double, so the candidates are ranked by their (nearly) exact scores
@@ -922,7 +936,10 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
 #pragma unroll
         for (int c = 0; c < G; c++) {
             re[c] = im[c] = 0.0;
-            row[c] = atoms + atom_row(ra, c0 + c < nc ? cand[c0 + c] : cand[0]) * T;
+            uint32_t ci;
+            if constexpr (kCandLds) ci = cand_lds[threadIdx.x >> 6][c0 + c < nc ? c0 + c : 0];
+            else ci = c0 + c < nc ? cand[c0 + c] : cand[0];
+            row[c] = atoms + atom_row(ra, ci) * T;
         }
         if ((T & 1) == 0) {
             // even T: rows are 16-byte aligned, two timepoints per load
@@ -963,7 +980,9 @@ __global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
         for (int c = 0; c < G; c++)
             if (c0 + c < nc) {
                 const double sc = re[c] * re[c] + im[c] * im[c];
-                const uint32_t ci = cand[c0 + c];
+                uint32_t ci;
+                if constexpr (kCandLds) ci = cand_lds[threadIdx.x >> 6][c0 + c];
+                else ci = cand[c0 + c];
                 if (sc > bs || (sc == bs && ci < bi)) {
                     bs = sc;
                     bi = ci;
