@@ -846,7 +846,7 @@ __device__ __forceinline__ size_t atom_row(const RescoreArgs &r, uint32_t gi)
 }
 
 template <int K, int CK>
-__global__ __launch_bounds__(kBlock) void rescore_kernel(RescoreArgs ra)
+__global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(RescoreArgs ra)
 {
     const float *part_s = ra.part_s;
     const uint32_t *part_i = ra.part_i;
