@@ -828,6 +828,17 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
             for (int m : kMs)
                 if (m * L >= l.w) { M = m; break; }
         L /= 2;
+        // small dictionaries: one atom per lane leaves most SIMDs idle and
+        // each lane a long per-TR chain; spread the atoms over more lanes
+        // (fewer states each, at least 8) until the launch has 2^17 lanes
+        while (M && L < 16 && l.natoms * (size_t)l.n_slices * (size_t)L < ((size_t)1 << 17)) {
+            int m2 = 0;
+            for (int m : kMs)
+                if (m * 2 * L >= l.w) { m2 = m; break; }
+            if (m2 < 8) break;
+            L *= 2;
+            M = m2;
+        }
     } else {
         const int mw = (l.w <= 128) ? 8 : 32;
         for (L = 1; L <= 16 && mw * L < l.w; L *= 2) {}
