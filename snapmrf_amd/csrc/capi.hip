@@ -69,7 +69,8 @@ __global__ void coef_kernel(const float *seq, int T, const float *params, size_t
         c.tmte = tr - te;
         c.s2p = (float)sin(2.0 * (double)ph);
         c.c2p = (float)cos(2.0 * (double)ph);
-        c.pad[0] = c.pad[1] = c.pad[2] = 0.f;
+        c.pad[0] = fa;   // the angle alpha * B1 itself (per-voxel B1 paths)
+        c.pad[1] = c.pad[2] = 0.f;
         coef[(size_t)j * T + n] = c;
         if (b1_scale) fa = fa * (1.0f / b1);
     }
@@ -482,6 +483,33 @@ void build_dtr(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<DTr> 
     }
 }
 
+// The same sequence as build_dtr in the fused kernels' per-TR record, B1 = 1,
+// with the flip angle in pad[0] (the Gauss-Newton trial passes form their own
+// RF angle alpha_n * B1 per voxel).
+void build_gn_coef(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<TrCoef> &out)
+{
+    std::vector<DTr> d;
+    build_dtr(h_mrf, nreps, NREPS, d);
+    out.resize(nreps);
+    for (size_t i = 0; i < nreps; i++) {
+        const double ph = (double)h_mrf[NREPS + i] * (M_PI / 180.0);
+        const double ph_next = i + 1 < nreps ? (double)h_mrf[NREPS + i + 1] * (M_PI / 180.0) : ph;
+        TrCoef c{};
+        c.ca2 = 1.f;
+        c.zr = d[i].cph;
+        c.zi = d[i].sph;
+        c.rr = (float)cos(ph - ph_next);
+        c.ri = (float)sin(ph - ph_next);
+        c.te = d[i].te;
+        c.tmte = d[i].t2;
+        c.tr = d[i].te + d[i].t2;
+        c.s2p = d[i].s2p;
+        c.c2p = d[i].c2p;
+        c.pad[0] = d[i].al;
+        out[i] = c;
+    }
+}
+
 constexpr float kFaUnit = (float)(60.0 * M_PI / 180.0);
 constexpr size_t kDiffScratchBudget = (size_t)4 << 30;   // bytes per launch batch
 
@@ -595,14 +623,25 @@ extern "C" int mrf_gn_refine(void *stream, const float *h_mrf, size_t nreps, siz
     hipStream_t s = (hipStream_t)stream;
     DTr *d_seq = nullptr;
     if (int rc = upload_dtr(h_mrf, nreps, NREPS, s, &d_seq)) return rc;
+    std::vector<TrCoef> coef;
+    build_gn_coef(h_mrf, nreps, NREPS, coef);
+    TrCoef *d_coef = nullptr;
     void *scratch = nullptr;
-    if (hipMallocAsync(&scratch, gn_scratch_bytes(V), s) != hipSuccess) {
+    if (hipMallocAsync((void **)&d_coef, nreps * sizeof(TrCoef), s) != hipSuccess ||
+        hipMallocAsync(&scratch, gn_scratch_bytes(V), s) != hipSuccess) {
+        (void)hipFreeAsync(d_coef, s);
         (void)hipFreeAsync(d_seq, s);
         return mrf_fail(MRF_E_NOMEM, "Gauss-Newton state (%zu bytes)", gn_scratch_bytes(V));
     }
-    int rc = launch_gn(d_seq, (int)nreps, nstates, reinterpret_cast<const float2 *>(d_sig), V,
-                       d_theta, iters, d_pd, d_cost, scratch, s);
+    int rc = 0;
+    if (hipMemcpyAsync(d_coef, coef.data(), nreps * sizeof(TrCoef), hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+        rc = mrf_fail(MRF_E_GPU, "Gauss-Newton coefficient upload failed");
+    if (!rc)
+        rc = launch_gn(d_seq, d_coef, (int)nreps, nstates, reinterpret_cast<const float2 *>(d_sig),
+                       V, d_theta, iters, d_pd, d_cost, scratch, s);
     (void)hipFreeAsync(scratch, s);
+    (void)hipFreeAsync(d_coef, s);
     (void)hipFreeAsync(d_seq, s);
     return rc;
 }

@@ -813,6 +813,92 @@ int launch_m(const EpgLaunch &l, const EpgArgs &args, dim3 grid, hipStream_t s)
     return 0;
 }
 
+// Gauss-Newton trial cost, one voxel per lane: the fused epg_ssfp recursion
+// at the voxel's trial (T1, T2, B0, B1) with its own RF angle alpha_n * B1
+// (sincosf per TR, as the tangent passes form it), folded against the
+// voxel's signal into s^H s and s^H x (float, summed in blocks of 32 TRs).
+template <int M>
+__global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float2 *__restrict__ sig,
+                                                          size_t V, float *rec, int stride)
+{
+    const size_t v = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    float *r = rec + v * (size_t)stride;
+    const float T1 = r[kGnRecTrial], T2 = r[kGnRecTrial + 1], B0 = r[kGnRecTrial + 2],
+                B1 = r[kGnRecTrial + 3];
+    if (!(T1 > 0.f && T2 > 0.f && B1 > 0.f && isfinite(T1) && isfinite(T2) && isfinite(B0) &&
+          isfinite(B1))) {
+        r[kGnRecCt] = INFINITY;
+        r[kGnRecHs] = r[kGnRecHsx] = r[kGnRecHsx + 1] = 0.f;
+        return;
+    }
+    FusedCtx x;
+    x.j = 0;
+    x.k1 = decay_k(T1);
+    x.k2 = decay_k(T2);
+    x.b0r = (double)B0 * 1e-3;
+    const TrCoef *__restrict__ cf = a.coef;
+    FusedState<M> s;
+    {
+        EpgState<M> s0;
+        epg_init(s0, a, 0, T1, T2);
+        const float2 z0 = make_float2(cf[0].zr, cf[0].zi);
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            s.fp[i] = s0.fp[i];
+            s.fm[i] = s0.fm[i];
+            s.z[i] = s0.z[i];
+        }
+#pragma unroll
+        for (int i = M; i < M + kUnroll; i++) s.fm[i] = make_float2(0.f, 0.f);
+        s.fp[0] = cmulc(z0, s.fp[0]);
+        s.fm[0] = cmul(z0, s.fm[0]);
+    }
+    auto coef_at = [&](int n) {
+        TrCoef c = cf[n];
+        float sa, ca;
+        sincosf(c.pad[0] * B1, &sa, &ca);
+        c.sa = sa;
+        c.ca = ca;
+        c.ca2 = 0.5f * (1.f + ca);
+        c.sa2 = 0.5f * (1.f - ca);
+        return c;
+    };
+    const float2 *xs = sig + v * (size_t)a.T;
+    float S = 0.f, Sr = 0.f, Si = 0.f, bS = 0.f, bR = 0.f, bI = 0.f;
+    auto fold = [&](float2 p, float2 q) {
+        bS += p.x * p.x + p.y * p.y;
+        bR += p.x * q.x + p.y * q.y;     // Re conj(p) q
+        bI += p.x * q.y - p.y * q.x;     // Im conj(p) q
+    };
+    int n = 0;
+    for (; n + kUnroll <= a.T; n += kUnroll) {
+        const float2 v0 = fused_step<M, 0, false, true, false, false>(s, coef_at(n), x, a);
+        const float2 v1 = fused_step<M, 1, false, true, false, false>(s, coef_at(n + 1), x, a);
+        fm_copy_down<M, 2>(s);
+        fold(v0, xs[n]);
+        fold(v1, xs[n + 1]);
+        if (((n + 2) & 31) == 0) {
+            S += bS;
+            Sr += bR;
+            Si += bI;
+            bS = bR = bI = 0.f;
+        }
+    }
+    for (; n < a.T; n++) {
+        const float2 v0 = fused_step<M, 0, false, true, false, false>(s, coef_at(n), x, a);
+        fm_copy_down<M, 1>(s);
+        fold(v0, xs[n]);
+    }
+    S += bS;
+    Sr += bR;
+    Si += bI;
+    r[kGnRecCt] = r[kGnRecXx] - (Sr * Sr + Si * Si) / S;
+    r[kGnRecHs] = S;
+    r[kGnRecHsx] = Sr;
+    r[kGnRecHsx + 1] = Si;
+}
+
 }  // namespace
 
 int launch_epg(const EpgLaunch &l, hipStream_t s)
@@ -938,6 +1024,38 @@ int launch_normalize(float2 *atoms, const double *norm2, size_t n, int T, hipStr
     normalize_kernel<<<dim3((unsigned)n), kBlock, 0, s>>>(atoms, norm2, n, T);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "normalize launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int launch_gn_trial(const TrCoef *coef, int T, int w, const float2 *sig, size_t V, float *rec,
+                    int stride, hipStream_t s)
+{
+    if (V == 0) return 0;
+    EpgArgs a{};
+    a.coef = coef;
+    a.T = T;
+    a.L = 1;
+    a.istar = -1;
+    a.jstar = 0;
+    a.init_alpha = (float)M_PI;   // init_rf_pulse(180 deg, 0 deg, TI = 40) (src/functions.cu:965-966)
+    a.init_phi = 0.f;
+    a.TI = 40.f;
+    const dim3 grid((unsigned)((V + kBlock - 1) / kBlock));
+    switch (w) {
+#define MRF_GN_TRIAL(M) \
+    case M: gn_trial_kernel<M><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride); break;
+        MRF_GN_TRIAL(4)
+        MRF_GN_TRIAL(8)
+        MRF_GN_TRIAL(12)
+        MRF_GN_TRIAL(16)
+        MRF_GN_TRIAL(20)
+        MRF_GN_TRIAL(24)
+        MRF_GN_TRIAL(28)
+        MRF_GN_TRIAL(30)
+        MRF_GN_TRIAL(32)
+#undef MRF_GN_TRIAL
+    default: return 1;
+    }
     return 0;
 }
 

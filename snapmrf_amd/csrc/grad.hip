@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 
 namespace mrf {
 
@@ -969,6 +970,14 @@ struct GnState {
     GnSums g;          // tangent-pass sums at th
 };
 
+static_assert(offsetof(GnState, trial) == kGnRecTrial * sizeof(float) &&
+                  offsetof(GnState, ct) == kGnRecCt * sizeof(float) &&
+                  offsetof(GnState, xx) == kGnRecXx * sizeof(float) &&
+                  offsetof(GnState, hS) == kGnRecHs * sizeof(float) &&
+                  offsetof(GnState, hSx) == kGnRecHsx * sizeof(float) &&
+                  sizeof(GnState) % sizeof(float) == 0,
+              "Gauss-Newton record layout");
+
 // group's voxel: one group of SUB lanes per voxel (or per list entry)
 template <int SUB>
 __global__ __launch_bounds__(64 * kWaves) void gn_init_kernel(GnArgs A)
@@ -1204,8 +1213,8 @@ size_t gn_scratch_bytes(size_t V)
 // tangent pass for the voxels that moved (compacted, so a rejected voxel
 // never waits in a wave for an accepted one).  The decisions are those of a
 // single-kernel loop; the passes get their own register budgets.
-int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
-              float *pd, float *cost, void *scratch, hipStream_t s)
+int launch_gn(const DTr *seq, const TrCoef *coef, int T, int w, const float2 *sig, size_t V,
+              float *theta, int iters, float *pd, float *cost, void *scratch, hipStream_t s)
 {
     if (V == 0) return 0;
     GnArgs A{};
@@ -1229,8 +1238,13 @@ int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *
     else gn_init_kernel<64><<<gg, gb, 0, s>>>(A);
     for (int it = 0; it < iters; it++) {
         gn_solve_kernel<<<vb, 256, 0, s>>>(A);
-        if (sub == 32) gn_eval_kernel<32><<<gg, gb, 0, s>>>(A);
-        else gn_eval_kernel<64><<<gg, gb, 0, s>>>(A);
+        // the trial cost: one voxel per lane where nstates allows, else
+        // one voxel per lane group
+        if (launch_gn_trial(coef, T, w, sig, V, reinterpret_cast<float *>(A.st),
+                            (int)(sizeof(GnState) / sizeof(float)), s)) {
+            if (sub == 32) gn_eval_kernel<32><<<gg, gb, 0, s>>>(A);
+            else gn_eval_kernel<64><<<gg, gb, 0, s>>>(A);
+        }
         gn_accept_kernel<<<vb, 256, 0, s>>>(A, it);
         if (it + 1 < iters) {
             if (sub == 32) gn_tangent_kernel<32><<<gg, gb, 0, s>>>(A);

@@ -25,7 +25,7 @@ struct TrCoef {
     float rr, ri;             // e^{i (phi_n - phi_{n+1})} (phi_T := phi_{T-1})
     float te, tr, tmte;       // TE_n, TR_n, TR_n - TE_n (float32, as the reference)
     float s2p, c2p;           // sin(2 phi_n), cos(2 phi_n) (ROA rotation)
-    float pad[3];
+    float pad[3];             // pad[0]: alpha * B1 itself
 };
 static_assert(sizeof(TrCoef) == 64, "TrCoef must be 64 bytes");
 

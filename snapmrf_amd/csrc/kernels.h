@@ -113,10 +113,18 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
                    float *grad_theta, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
 int launch_crb(const DTr *seq, int T, int w, const float *theta, size_t n, const float *wts,
                double *loss, float *grad_seq, float fa_unit, float *scratch, hipStream_t s);
+// Gauss-Newton state record: float offsets of the trial step (T1, T2, B0,
+// B1), its cost, |x|^2 and its sums s^H s, s^H x (re, im).
+constexpr int kGnRecTrial = 4, kGnRecCt = 9, kGnRecXx = 10, kGnRecHs = 14, kGnRecHsx = 15;
+// Trial cost of every voxel by the one-voxel-per-lane EPG-SSFP recursion
+// (coef: [T] coefficients at B1 = 1 with the flip angle in pad[0]); returns 1
+// when nstates is not a state count it is compiled for.
+int launch_gn_trial(const TrCoef *coef, int T, int w, const float2 *sig, size_t V, float *rec,
+                    int stride, hipStream_t s);
 // scratch bytes of launch_gn for V voxels
 size_t gn_scratch_bytes(size_t V);
-int launch_gn(const DTr *seq, int T, int w, const float2 *sig, size_t V, float *theta, int iters,
-              float *pd, float *cost, void *scratch, hipStream_t s);
+int launch_gn(const DTr *seq, const TrCoef *coef, int T, int w, const float2 *sig, size_t V,
+              float *theta, int iters, float *pd, float *cost, void *scratch, hipStream_t s);
 
 // B1 slice per voxel: argmin_k |b1map[v] - slice_b1[k]| (strict <, init 10).
 int launch_b1_select(const float *b1map, const float *params, size_t natoms, int n_slices,
