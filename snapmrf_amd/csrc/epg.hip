@@ -817,23 +817,30 @@ int launch_m(const EpgLaunch &l, const EpgArgs &args, dim3 grid, hipStream_t s)
 // at the voxel's trial (T1, T2, B0, B1) with its own RF angle alpha_n * B1
 // (sincosf per TR, as the tangent passes form it), folded against the
 // voxel's signal into s^H s and s^H x (float, summed in blocks of 32 TRs).
-template <int M>
+// MULTI: each voxel on a lane pair (j = 0, 1) holding M states each (the
+// generator's multi-lane form; TRUNC when 2M > w), which doubles the waves of
+// a launch that would otherwise leave most SIMDs with one latency-bound wave.
+template <int M, bool MULTI, bool TRUNC>
 __global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float2 *__restrict__ sig,
                                                           size_t V, float *rec, int stride)
 {
-    const size_t v = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t gl = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t v = MULTI ? gl >> 1 : gl;
+    const int j = MULTI ? (int)(gl & 1) : 0;
     if (v >= V) return;
     float *r = rec + v * (size_t)stride;
     const float T1 = r[kGnRecTrial], T2 = r[kGnRecTrial + 1], B0 = r[kGnRecTrial + 2],
                 B1 = r[kGnRecTrial + 3];
     if (!(T1 > 0.f && T2 > 0.f && B1 > 0.f && isfinite(T1) && isfinite(T2) && isfinite(B0) &&
           isfinite(B1))) {
-        r[kGnRecCt] = INFINITY;
-        r[kGnRecHs] = r[kGnRecHsx] = r[kGnRecHsx + 1] = 0.f;
+        if (j == 0) {
+            r[kGnRecCt] = INFINITY;
+            r[kGnRecHs] = r[kGnRecHsx] = r[kGnRecHsx + 1] = 0.f;
+        }
         return;
     }
     FusedCtx x;
-    x.j = 0;
+    x.j = j;
     x.k1 = decay_k(T1);
     x.k2 = decay_k(T2);
     x.b0r = (double)B0 * 1e-3;
@@ -841,7 +848,7 @@ __global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float
     FusedState<M> s;
     {
         EpgState<M> s0;
-        epg_init(s0, a, 0, T1, T2);
+        epg_init(s0, a, j, T1, T2);
         const float2 z0 = make_float2(cf[0].zr, cf[0].zi);
 #pragma unroll
         for (int i = 0; i < M; i++) {
@@ -873,8 +880,8 @@ __global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float
     };
     int n = 0;
     for (; n + kUnroll <= a.T; n += kUnroll) {
-        const float2 v0 = fused_step<M, 0, false, true, false, false>(s, coef_at(n), x, a);
-        const float2 v1 = fused_step<M, 1, false, true, false, false>(s, coef_at(n + 1), x, a);
+        const float2 v0 = fused_step<M, 0, MULTI, true, false, TRUNC>(s, coef_at(n), x, a);
+        const float2 v1 = fused_step<M, 1, MULTI, true, false, TRUNC>(s, coef_at(n + 1), x, a);
         fm_copy_down<M, 2>(s);
         fold(v0, xs[n]);
         fold(v1, xs[n + 1]);
@@ -886,10 +893,11 @@ __global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float
         }
     }
     for (; n < a.T; n++) {
-        const float2 v0 = fused_step<M, 0, false, true, false, false>(s, coef_at(n), x, a);
+        const float2 v0 = fused_step<M, 0, MULTI, true, false, TRUNC>(s, coef_at(n), x, a);
         fm_copy_down<M, 1>(s);
         fold(v0, xs[n]);
     }
+    if (j != 0) return;   // the sample is state 0, on lane 0 of the pair
     S += bS;
     Sr += bR;
     Si += bI;
@@ -1035,25 +1043,36 @@ int launch_gn_trial(const TrCoef *coef, int T, int w, const float2 *sig, size_t 
     a.coef = coef;
     a.T = T;
     a.L = 1;
+    a.lshift = 0;
     a.istar = -1;
     a.jstar = 0;
     a.init_alpha = (float)M_PI;   // init_rf_pulse(180 deg, 0 deg, TI = 40) (src/functions.cu:965-966)
     a.init_phi = 0.f;
     a.TI = 40.f;
+    if (w >= 9 && w <= 32) {
+        // lane pairs of M = 8, 12 or 16 states
+        const int M = w <= 16 ? 8 : (w <= 24 ? 12 : 16);
+        a.L = 2;
+        a.lshift = 1;
+        const bool trunc = 2 * M > w;
+        a.istar = trunc ? (w - 1) % M : -1;
+        a.jstar = (w - 1) / M;
+        const dim3 grid((unsigned)((2 * V + kBlock - 1) / kBlock));
+#define MRF_GN_TRIAL2(MM)                                                                    \
+    if (M == MM) {                                                                           \
+        if (trunc) gn_trial_kernel<MM, true, true><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride); \
+        else gn_trial_kernel<MM, true, false><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride);     \
+    }
+        MRF_GN_TRIAL2(8)
+        MRF_GN_TRIAL2(12)
+        MRF_GN_TRIAL2(16)
+#undef MRF_GN_TRIAL2
+        return 0;
+    }
     const dim3 grid((unsigned)((V + kBlock - 1) / kBlock));
     switch (w) {
-#define MRF_GN_TRIAL(M) \
-    case M: gn_trial_kernel<M><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride); break;
-        MRF_GN_TRIAL(4)
-        MRF_GN_TRIAL(8)
-        MRF_GN_TRIAL(12)
-        MRF_GN_TRIAL(16)
-        MRF_GN_TRIAL(20)
-        MRF_GN_TRIAL(24)
-        MRF_GN_TRIAL(28)
-        MRF_GN_TRIAL(30)
-        MRF_GN_TRIAL(32)
-#undef MRF_GN_TRIAL
+    case 4: gn_trial_kernel<4, false, false><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride); break;
+    case 8: gn_trial_kernel<8, false, false><<<grid, kBlock, 0, s>>>(a, sig, V, rec, stride); break;
     default: return 1;
     }
     return 0;

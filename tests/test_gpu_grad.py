@@ -179,6 +179,42 @@ def test_gauss_newton_recovers_off_grid_truths(gpu, V):
     assert np.allclose(pd, want_pd, rtol=1e-3)
 
 
+@pytest.mark.parametrize("w", [10, 16, 20, 30, 32, 33])
+def test_gauss_newton_cost_is_that_of_the_returned_theta(gpu, w):
+    """Noisy signals, three iterations: the last recorded cost and the proton
+    density are those of the returned theta, recomputed from the Jacobian
+    kernel's signal there in FP64 (|x|^2 - |s^H x|^2 / |s|^2 and
+    |s^H x| / |s|).  The state counts cover each form of the trial pass: lane
+    pairs of 8, 12 and 16 states with and without truncation, and w = 33 on
+    the generic evaluation."""
+    T, V, iters = 300, 200, 3
+    raw = _raw(T, seed=13)
+    rng = np.random.default_rng(w)
+    truth = np.stack([rng.uniform(300, 2500, V), rng.uniform(30, 300, V),
+                      rng.uniform(-30, 30, V), rng.uniform(0.85, 1.15, V)], 1).astype(np.float32)
+    s, _ = _jac(gpu, raw, truth, w)
+    scale = np.linalg.norm(s, axis=1, keepdims=True) / np.sqrt(T)
+    noise = (rng.standard_normal((V, T)) + 1j * rng.standard_normal((V, T))) * 0.05 * scale
+    x = (s + noise).astype(np.complex64)
+    start = truth * np.array([1.04, 0.96, 1.0, 1.0], np.float32) + np.array([0, 0, 1.0, 0.02], np.float32)
+    d_th = gpu.from_numpy(start.copy()).cuda()
+    d_pd = gpu.empty(V, device="cuda")
+    d_cost = gpu.empty((V, iters + 1), device="cuda")
+    sm.gn_refine(raw, gpu.from_numpy(x).cuda(), V, w, iters, d_th, d_pd=d_pd, d_cost=d_cost)
+    gpu.cuda.synchronize()
+    th, pd, cost = d_th.cpu().numpy(), d_pd.cpu().numpy(), d_cost.cpu().numpy()
+    assert np.all(np.diff(cost, axis=1) <= 0.0)
+    assert np.all(cost[:, -1] < cost[:, 0])
+    s1, _ = _jac(gpu, raw, th, w)
+    s64, x64 = s1.astype(np.complex128), x.astype(np.complex128)
+    sx = np.abs(np.sum(np.conj(s64) * x64, axis=1))
+    ss = np.sum(np.abs(s64) ** 2, axis=1)
+    xx = np.sum(np.abs(x64) ** 2, axis=1)
+    want = xx - sx ** 2 / ss
+    assert np.all(np.abs(cost[:, -1] - want) <= 1e-5 * xx + 1e-3 * want)
+    assert np.allclose(pd, sx / np.sqrt(ss), rtol=1e-4)
+
+
 def test_crb_loss_and_gradient(gpu):
     """CRB loss per tissue against the FP64 oracle, and its flip-angle and TR
     gradients against FP64 central differences of the oracle loss on 32
