@@ -275,8 +275,9 @@ int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, size_t NREPS
  * the matched grid point; out: refined) against d_sig [V][nreps], iters
  * iterations; the cost |x|^2 - |s^H x|^2/|s|^2 never increases.  d_pd [V]
  * (optional) = |s^H x|/|s|; d_cost [V][iters + 1] (optional) cost history.
- * The 4x4 normal equations are solved in FP64.  Allocates a 240-byte state
- * record per voxel on the stream. */
+ * The 4x4 normal equations are solved in FP64.  A start outside the domain
+ * (T1, T2, B1 > 0, all four finite) is returned unchanged with a NaN cost and
+ * proton density.  Allocates a 240-byte state record per voxel on the stream. */
 int mrf_gn_refine(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
                   const mrf_complex *d_sig, size_t V, int nstates, int iters, float *d_theta,
                   float *d_pd, float *d_cost);

@@ -994,15 +994,31 @@ __global__ __launch_bounds__(64 * kWaves) void gn_init_kernel(GnArgs A)
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
-    const GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), x, k, coef_lds, read_lds);
+    // a start outside the domain stays where it is: the pass runs at a valid
+    // point (the group keeps one control flow) and its sums are dropped, so
+    // the solve finds a singular system and every trial is rejected
+    const bool ok = theta_ok(th);
+    float tp[4] = {1000.f, 100.f, 0.f, 1.f};
+    if (ok)
+#pragma unroll
+        for (int d = 0; d < 4; d++) tp[d] = th[d];
+    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(tp), x, k, coef_lds, read_lds);
     if (k == 0) {
         GnState &st = A.st[v];
-        const float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
+        float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
+        if (!ok) {
+            const float2 z = make_float2(0.f, 0.f);
+            g.S = 1.f;
+            g.Sx = z;
+            for (int d = 0; d < 4; d++) g.U[d] = g.W[d] = z;
+            for (int i = 0; i < 10; i++) g.D[i] = z;
+            c = NAN;
+        }
 #pragma unroll
         for (int d = 0; d < 4; d++) st.th[d] = th[d];
         st.c = c;
         st.xx = xx;
-        st.S = g.S;
+        st.S = ok ? g.S : 0.f;
         st.Sx[0] = g.Sx.x;
         st.Sx[1] = g.Sx.y;
         st.lam = 1e-3;

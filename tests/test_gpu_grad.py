@@ -215,6 +215,48 @@ def test_gauss_newton_cost_is_that_of_the_returned_theta(gpu, w):
     assert np.allclose(pd, sx / np.sqrt(ss), rtol=1e-4)
 
 
+def test_gauss_newton_edge_voxels(gpu):
+    """A zero signal keeps its start with cost 0 and proton density 0; starts
+    outside the domain (T1 < 0, T2 NaN, B1 = 0) come back unchanged with NaN
+    cost and proton density; the other voxels' results are bit for bit those
+    of a launch without the edge voxels."""
+    T, w, iters, V = 200, 24, 4, 12
+    raw = _raw(T, seed=6)
+    rng = np.random.default_rng(9)
+    truth = np.stack([rng.uniform(300, 2500, V), rng.uniform(30, 300, V),
+                      rng.uniform(-30, 30, V), rng.uniform(0.85, 1.15, V)], 1).astype(np.float32)
+    s, _ = _jac(gpu, raw, truth, w)
+    x = (s * 0.8).astype(np.complex64)
+    start = (truth * np.float32(1.03)).astype(np.float32)
+
+    def run(xv, th0):
+        n = len(th0)
+        d_th = gpu.from_numpy(th0.copy()).cuda()
+        d_pd = gpu.empty(n, device="cuda")
+        d_cost = gpu.empty((n, iters + 1), device="cuda")
+        sm.gn_refine(raw, gpu.from_numpy(xv).cuda(), n, w, iters, d_th, d_pd=d_pd, d_cost=d_cost)
+        gpu.cuda.synchronize()
+        return d_th.cpu().numpy(), d_pd.cpu().numpy(), d_cost.cpu().numpy()
+
+    edge = [1, 4, 6, 9]
+    keep = [i for i in range(V) if i not in edge]
+    th_ref, pd_ref, cost_ref = run(np.ascontiguousarray(x[keep]), np.ascontiguousarray(start[keep]))
+    x_e, start_e = x.copy(), start.copy()
+    x_e[1] = 0
+    start_e[4, 0] = -5.0
+    start_e[6, 1] = np.nan
+    start_e[9, 3] = 0.0
+    th, pd, cost = run(x_e, start_e)
+    assert np.array_equal(th[1], start_e[1]) and pd[1] == 0.0 and np.all(cost[1] == 0.0)
+    for i in (4, 6, 9):
+        assert np.array_equal(th[i], start_e[i], equal_nan=True)
+        assert np.isnan(pd[i]) and np.all(np.isnan(cost[i]))
+    assert np.array_equal(th[keep], th_ref)
+    assert np.array_equal(pd[keep], pd_ref)
+    assert np.array_equal(cost[keep], cost_ref)
+    assert np.all(cost_ref[:, -1] < cost_ref[:, 0])
+
+
 def test_crb_loss_and_gradient(gpu):
     """CRB loss per tissue against the FP64 oracle, and its flip-angle and TR
     gradients against FP64 central differences of the oracle loss on 32
