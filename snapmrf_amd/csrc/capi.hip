@@ -224,7 +224,7 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                             size_t V, const float *d_b1map, float *d_maps, int32_t *d_idx)
 {
     mrf_clear_error();
-    if (!d_atoms || !d_params || !d_maps || (V && !d_sig))
+    if (!d_atoms || !d_params || (V && (!d_sig || !d_maps)))
         return mrf_fail(MRF_E_ARG, "null pointer argument");
     if (T == 0 || T > 0x7fffffff) return mrf_fail(MRF_E_ARG, "bad T");
     if (natoms == 0 || NATOMS == 0 || NATOMS % natoms)
@@ -276,7 +276,7 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     const int echo = echo_id(echo_type);
     if (echo < 0)
         return mrf_fail(MRF_E_ECHO, "unknown echo type '%s'", echo_type ? echo_type : "(null)");
-    if (!h_mrf || !d_params || !d_maps || (V && !d_sig))
+    if (!h_mrf || !d_params || (V && (!d_sig || !d_maps)))
         return mrf_fail(MRF_E_ARG, "null pointer argument");
     if (nreps == 0 || nreps > NREPS) return mrf_fail(MRF_E_ARG, "need 0 < nreps <= NREPS");
     if (nreps > 0x7fffffff) return mrf_fail(MRF_E_ARG, "nreps too large");

@@ -257,6 +257,15 @@ def test_gauss_newton_edge_voxels(gpu):
     assert np.all(cost_ref[:, -1] < cost_ref[:, 0])
 
 
+def test_gauss_newton_zero_voxels(gpu):
+    """V = 0 returns without launching (empty signal and theta tensors)."""
+    raw = _raw(50)
+    d_sig = gpu.empty((0, 50), dtype=gpu.complex64, device="cuda")
+    d_th = gpu.empty((0, 4), device="cuda")
+    sm.gn_refine(raw, d_sig, 0, 16, 3, d_th)
+    gpu.cuda.synchronize()
+
+
 def test_crb_loss_and_gradient(gpu):
     """CRB loss per tissue against the FP64 oracle, and its flip-angle and TR
     gradients against FP64 central differences of the oracle loss on 32

@@ -93,6 +93,26 @@ def test_config1_fp32_vs_fp64_oracle(gpu):
     _check_against_oracle(maps, idx, atoms, params, sig)
 
 
+@pytest.mark.parametrize("mode", [sm.MRF_MATCH_FP32, sm.MRF_MATCH_BF16_TOP32, sm.MRF_MATCH_SPLIT2])
+def test_zero_voxels_is_a_no_op(gpu, mode):
+    """V = 0 returns without launching (empty signal and map tensors)."""
+    T, A = 37, 300
+    rng = np.random.default_rng(5)
+    atoms = (rng.normal(size=(A, T)) + 1j * rng.normal(size=(A, T))).astype(np.complex64)
+    params = np.ones((A, 4), np.float32)
+    d_atoms, d_params = dev(gpu, atoms), dev(gpu, params)
+    d_hi = d_lo = None
+    if mode != sm.MRF_MATCH_FP32:
+        ld = sm.lowp_ld(T)
+        d_hi = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+        d_lo = gpu.empty((A, ld), dtype=gpu.int32, device="cuda")
+        sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo)
+    d_sig = gpu.empty((0, T), dtype=gpu.complex64, device="cuda")
+    d_maps = gpu.empty((0, 5), device="cuda")
+    sm.match_ex(mode, d_atoms, d_params, T, A, A, d_sig, 0, d_maps, d_lowp=d_hi, d_lowp_lo=d_lo)
+    gpu.cuda.synchronize()
+
+
 @pytest.mark.parametrize("T,V", [(1, 7), (3, 64), (15, 33), (37, 1), (37, 129), (16, 300), (1000, 257)])
 def test_ragged_shapes(gpu, T, V):
     rng = np.random.default_rng(T * 1000 + V)
