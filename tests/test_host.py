@@ -4,6 +4,7 @@ parser (src/functions.cu:196-414), rfpulse CSV reader (:100-158), RA I/O
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -174,3 +175,12 @@ def test_cli_usage_and_errors(tmp_path):
     assert r.returncode == 70 and "Number of atoms is 0" in r.stderr
     r = subprocess.run([sm.CLI_PATH, "--precision", "fp8"], capture_output=True, text=True)
     assert r.returncode == 64
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The measured path runs on one GPU; --gpus above 1 exits with a message
+    before touching a device."""
+    root = os.path.dirname(os.path.dirname(HEADER_H))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, cwd=root, timeout=300)
+    assert r.returncode == 2 and "one GPU" in (r.stdout + r.stderr)
