@@ -35,43 +35,51 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seq_raw, params, natoms, T, w, budget_s=8.0):
+def cpu_baseline(seq_raw, params, natoms, T, w, rows_idx, gpu_rows, slice_atoms, slice_sig,
+                 gpu_idx, budget_s=10.0):
     """FP64 oracle (C, OpenMP over all host cores) on a bounded seeded sample
-    of the same workload: EPG atoms/s and matching voxels/s."""
+    of the same workload.  Generation: the atoms rows_idx of B1 slice 0
+    (whose GPU rows are gpu_rows, compared with the oracle's), repeated until
+    about budget_s of CPU work.  Matching: slice_sig voxels against every atom
+    of slice 0 (slice_atoms, the GPU's complex64 dictionary rows), compared
+    with the GPU's picks gpu_idx."""
     import numpy as np
     import oracle
     seq = oracle.seq_prepare("epg_ssfp", seq_raw)
-    rng = np.random.default_rng(0)
-    n = 64
-    sub = params[np.sort(rng.choice(natoms, n, replace=False))]
+    sub = params[rows_idx]
+    n = len(rows_idx)
     t0 = oracle.wall()
-    oracle.epg("epg_ssfp", seq, sub, n, w)
-    dt = oracle.wall() - t0
-    n2 = int(max(n, min(20000, n * budget_s / max(dt, 1e-6))))
-    sub = params[np.sort(rng.choice(natoms, n2, replace=False))]
-    t0 = oracle.wall()
-    atoms = oracle.epg("epg_ssfp", seq, sub, n2, w)
-    dt_gen = oracle.wall() - t0
-    # matching: a few voxels against the whole 174,291-atom slice is too slow;
-    # time voxels x sampled atoms and scale to the full slice
-    a32 = atoms.astype(np.complex64)
-    V = 32
-    sig = (a32[rng.integers(0, n2, V)] * 0.8).astype(np.complex64)
-    t0 = oracle.wall()
-    oracle.match_slice(a32, sig)
-    dt_m = oracle.wall() - t0
-    vox_per_s = V / (dt_m * natoms / n2)
+    atoms = oracle.epg("epg_ssfp", seq, sub, n, w)
+    done, dt_gen = n, oracle.wall() - t0
+    rel = float(np.max(np.abs(gpu_rows - atoms).max(axis=1) / np.abs(atoms).max(axis=1)))
+    del atoms
+    reps = 0
+    while dt_gen < budget_s and reps < 64:
+        t0 = oracle.wall()
+        oracle.epg("epg_ssfp", seq, sub, n, w)
+        dt_gen += oracle.wall() - t0
+        done += n
+        reps += 1
+    match = {}
+    if slice_sig is not None:
+        t0 = oracle.wall()
+        ridx, _, _ = oracle.match_slice(slice_atoms, slice_sig)
+        dt_m = oracle.wall() - t0
+        match = dict(match_value=len(slice_sig) / dt_m, match_unit="voxels/s",
+                     match_argmax_equal_gpu=float(np.mean(ridx == gpu_idx)),
+                     match_sample=f"{len(slice_sig)} voxels of B1 slice 0 against all "
+                                  f"{len(slice_atoms)} atoms of the slice ({dt_m:.1f} s)")
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
             cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
-    return dict(value=n2 / dt_gen, unit="atoms/s", cores=oracle.num_threads(), kind="port",
+    return dict(value=done / dt_gen, unit="atoms/s", cores=oracle.num_threads(), kind="port",
                 cpu_model=cpu_model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
-                sample=f"epg_ssfp FP64 oracle, {n2} seeded atoms of the grid x T={T} x w={w}; "
-                       f"matching {V} voxels x {n2} atoms scaled to {natoms} atoms/slice",
-                match_value=vox_per_s, match_unit="voxels/s")
+                sample=f"epg_ssfp FP64 oracle: {n} seeded atoms of B1 slice 0 x T={T} x w={w}, "
+                       f"{done} atoms in {dt_gen:.1f} s",
+                gpu_atoms_max_rel_err=rel, gpu_atoms_checked=n, **match)
 
 
 def make_signals(torch, np, synth, d_atoms, params, natoms, T, nx=240, ny=240, snr=25.0, seed=0):
@@ -92,7 +100,7 @@ def make_signals(torch, np, synth, d_atoms, params, natoms, T, nx=240, ny=240, s
     noise = torch.complex(torch.randn(V, T, device="cuda", generator=g),
                           torch.randn(V, T, device="cuda", generator=g))
     d_sig = (clean + noise * (rms / (snr * 2 ** 0.5))[:, None]).to(torch.complex64).contiguous()
-    return d_sig, torch.from_numpy(b1).cuda(), truth, V
+    return d_sig, torch.from_numpy(b1).cuda(), truth, V, ks
 
 
 def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_atoms, d_norm2,
@@ -103,7 +111,7 @@ def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_
     log("profile: EPG launch done")
     sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms, d_norm2=d_norm2,
                stream=stream)
-    d_sig, d_b1, truth, V = make_signals(torch, np, synth, d_atoms, params, natoms, T)
+    d_sig, d_b1, truth, V, _ = make_signals(torch, np, synth, d_atoms, params, natoms, T)
     d_maps = torch.empty((V, 5), device="cuda")
     sm.match_ex(sm.MRF_MATCH_FP32, d_atoms, d_params, T, A, natoms, d_sig, V, d_maps,
                 d_b1map=d_b1, stream=stream)
@@ -126,16 +134,25 @@ def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_
     return 0
 
 
-GRID_LOWP = ("100:20:2000+2300:300:5000", "20:5:100+110:10:300+400:200:2000", "-40:4:40",
-             "0.8:0.1:1.2")
 PEAK_BF16 = 2500.0         # TFLOP/s, MI355X dense BF16 MFMA
+LOWP_PER_SLICE = 100000
+
+
+def lowp_params(sm, np):
+    """The 500,000-atom precision workload: the first 100,000 atoms (the
+    reference order: T1 fastest, then T2, B0) of each of the five B1 slices
+    of the 871,455-atom grid, i.e. B0 -40 .. 6 Hz at 2 Hz steps."""
+    params, A, natoms = sm.build_params(*GRID)
+    rows = (np.arange(A // natoms)[:, None] * natoms
+            + np.arange(LOWP_PER_SLICE)[None, :]).ravel()
+    return np.ascontiguousarray(params[rows]), len(rows), LOWP_PER_SLICE
 
 
 def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle_voxels=256):
-    """BF16 storage + BF16 MFMA + rescore of the top 8/16/32 candidates, and
-    the split-BF16 forms, against the FP32 matcher on 240x240x8 voxels; map
-    errors against the FP64 oracle on a seeded voxel subset."""
-    params, A, natoms = sm.build_params(*GRID_LOWP)
+    """BF16 storage + BF16 MFMA + rescore of the top 8/16/32/64 candidates,
+    and the split-BF16 forms, against the FP32 matcher on 240x240x8 voxels;
+    map errors against the FP64 oracle on a seeded voxel subset."""
+    params, A, natoms = lowp_params(sm, np)
     log(f"precision form: {A} atoms ({A // natoms} slices x {natoms}), 460800 voxels")
     d_params = torch.from_numpy(params).cuda()
     d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
@@ -146,8 +163,8 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     prep_ms = timed(lambda: sm.lowp_prepare(d_atoms, T, A, d_hi, d_lo, stream=stream), 1)
     sigs, b1s, truths = [], [], []
     for sl in range(8):
-        d_sig, d_b1, truth, V1 = make_signals(torch, np, synth, d_atoms, params, natoms, T,
-                                              seed=sl)
+        d_sig, d_b1, truth, V1, _ = make_signals(torch, np, synth, d_atoms, params, natoms, T,
+                                                 seed=sl)
         sigs.append(d_sig)
         b1s.append(d_b1)
         truths.append(truth)
@@ -156,16 +173,20 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     truth = np.concatenate(truths)
     del sigs, b1s
     V = d_sig.shape[0]
-    res = {"workload": f"epg_ssfp {A} atoms (B0 -40:4:40) x T={T}, 240x240x8 = {V} voxels, "
+    res = {"workload": f"epg_ssfp {A} atoms (first {natoms} of each B1 slice of the 871455-atom "
+                       f"grid: B0 -40..6 Hz at 2 Hz) x T={T}, 240x240x8 = {V} voxels, "
                        f"B1-map slice selection", "atoms": A, "voxels": V,
            "dictionary_bytes": {"c64": A * T * 8, "bf16": A * ld * 4, "split_bf16": A * ld * 8},
            "lowp_prepare_ms": prep_ms}
     out = {}
+    r32 = sm.MRF_MATCH_RESCORE_FP32
     for name, mode, k in (("fp32", sm.MRF_MATCH_FP32, 1),
                           ("bf16_top8", sm.MRF_MATCH_BF16_TOP8, steps),
                           ("bf16_top16", sm.MRF_MATCH_BF16_TOP16, steps),
                           ("bf16_top32", sm.MRF_MATCH_BF16_TOP32, steps),
                           ("bf16_top64", sm.MRF_MATCH_BF16_TOP64, steps),
+                          ("bf16_top32_fp32_rescore", sm.MRF_MATCH_BF16_TOP32 | r32, steps),
+                          ("bf16_top64_fp32_rescore", sm.MRF_MATCH_BF16_TOP64 | r32, steps),
                           ("split2", sm.MRF_MATCH_SPLIT2, steps),
                           ("split3", sm.MRF_MATCH_SPLIT3, steps)):
         d_maps = torch.empty((V, 5), device="cuda")
@@ -184,7 +205,8 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                      "truth_agree": float(np.mean(out[name][1] == truth))}
         log(f"precision form {name}: {ms:.1f} ms/call")
     i32 = out["fp32"][1]
-    lowp_names = ("bf16_top8", "bf16_top16", "bf16_top32", "bf16_top64", "split2", "split3")
+    lowp_names = ("bf16_top8", "bf16_top16", "bf16_top32", "bf16_top64", "bf16_top32_fp32_rescore",
+                  "bf16_top64_fp32_rescore", "split2", "split3")
     for name in lowp_names:
         res[name]["argmax_agree_fp32"] = float(np.mean(out[name][1] == i32))
         res[name]["speedup_vs_fp32"] = res[name]["voxels_per_s"] / res["fp32"]["voxels_per_s"]
@@ -208,11 +230,20 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
         "argmax_agree_whole_dictionary": float(np.mean(st_idx == out["bf16_top32"][1])),
         "argmax_agree_fp32": float(np.mean(st_idx == out["fp32"][1]))}
     log(f"precision form bf16_top32 streamed (generation included): {st_ms:.1f} ms")
-    res["roofline_bf16"] = {"bound": "mfma", "achieved": res["bf16_top16"]["tflops_algorithmic"],
+    # the precision form's headline: the fastest one-product BF16 mode that
+    # meets the >= 99.9 % agreement bar
+    kernels = {"bf16_top8": "match_lowp_kernel<1,32,3,2> + rescore_kernel<8,24>",
+               "bf16_top16": "match_lowp_kernel<1,32,3,2> + rescore_kernel<16,24>",
+               "bf16_top32": "match_lowp_kernel<1,32,6,1> + rescore_kernel<32,48>",
+               "bf16_top64": "match_lowp_kernel<1,16,8,1> + rescore_kernel<64,64>"}
+    ok = [n for n in kernels if res[n]["argmax_agree_fp32"] >= 0.999]
+    best = max(ok, key=lambda n: res[n]["voxels_per_s"]) if ok else "bf16_top64"
+    res["headline_mode"] = best
+    res["roofline_bf16"] = {"bound": "mfma", "achieved": res[best]["tflops_algorithmic"],
                             "peak": PEAK_BF16, "unit": "TFLOP/s",
-                            "frac": res["bf16_top16"]["tflops_algorithmic"] / PEAK_BF16,
-                            "traffic": measured_traffic("match_bf16"),
-                            "kernel": "match_lowp_kernel<1> + rescore_kernel<16>"}
+                            "frac": res[best]["tflops_algorithmic"] / PEAK_BF16,
+                            "traffic": measured_traffic("match_" + best),
+                            "kernel": kernels[best], "mode": best}
     # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path, and
     # on a sample of the voxels where a low-precision pick differs from the
     # FP32 pick: which of the two the exact (FP64) argmax chooses.
@@ -238,7 +269,12 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
         sub = np.sort(rng.choice(V, oracle_voxels, replace=False))
         t0 = time.time()
         _, ref = fp64_match(sub)
-        log(f"FP64 oracle on {oracle_voxels} voxels: {time.time() - t0:.1f} s")
+        dt_cpu = time.time() - t0
+        log(f"FP64 oracle on {oracle_voxels} voxels: {dt_cpu:.1f} s")
+        res["cpu_fp64_match"] = {"voxels_per_s": oracle_voxels / dt_cpu, "voxels": oracle_voxels,
+                                 "cores": oracle.num_threads(), "kind": "port",
+                                 "sample": f"{oracle_voxels} seeded voxels, each against its "
+                                           f"B1 slice ({natoms} atoms), FP64 oracle"}
         errs = {}
         for name in ("fp32",) + lowp_names:
             m = out[name][0][sub]
@@ -254,7 +290,9 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
             if len(diff) == 0:
                 arb[name] = {"disagree": 0}
                 continue
-            samp = np.sort(rng.choice(diff, min(len(diff), 256), replace=False))
+            # every disagreement, or a seeded sample of 4,096 where there are
+            # more (about 20 s of FP64 matching on 16 host threads)
+            samp = np.sort(rng.choice(diff, min(len(diff), 4096), replace=False))
             t0 = time.time()
             ridx, _ = fp64_match(samp)
             lowp_right = float(np.mean(out[name][1][samp] == ridx))
@@ -719,6 +757,10 @@ def main() -> int:
     gen()   # leave the dictionary normalised
     torch.cuda.synchronize()
     log(f"EPG kernel: {kern_ms:.2f} ms/launch")
+    # rows of B1 slice 0 that the CPU baseline recomputes with the FP64 oracle
+    cb_rows = np.sort(np.random.default_rng(0).choice(natoms, min(natoms, 20000), replace=False))
+    cb_gpu = None if args.no_cpu else d_atoms[torch.from_numpy(cb_rows).cuda()].cpu().numpy()
+    cb_slice = cb_sig = cb_idx = None
     flops = float(A) * T * w * FLOP_PER_STATE_SSFP
     achieved = flops / (kern_ms * 1e-3) / 1e12
 
@@ -739,7 +781,7 @@ def main() -> int:
     }
 
     if not args.skip_match:
-        d_sig, d_b1, truth, V = make_signals(torch, np, synth, d_atoms, params, natoms, T)
+        d_sig, d_b1, truth, V, vox_ks = make_signals(torch, np, synth, d_atoms, params, natoms, T)
         d_maps = torch.empty((V, 5), device="cuda")
         d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
 
@@ -754,6 +796,13 @@ def main() -> int:
         match_ms = timed(match, msteps)
         log(f"matching fp32: {match_ms:.1f} ms/step")
         idx = d_idx.cpu().numpy()
+        if not args.no_cpu:
+            # 256 voxels whose B1 map selects slice 0, for the CPU matcher
+            sel = np.sort(np.random.default_rng(1).choice(np.nonzero(vox_ks == 0)[0], 256,
+                                                           replace=False))
+            cb_sig = d_sig[torch.from_numpy(sel).cuda()].cpu().numpy()
+            cb_idx = idx[sel]
+            cb_slice = d_atoms[:natoms].cpu().numpy()
         mflops = 8.0 * T * natoms * V
         ach_m = mflops / (match_ms * 1e-3) / 1e12
         out["match_voxels_per_s"] = V / (match_ms * 1e-3)
@@ -832,10 +881,14 @@ def main() -> int:
                                                args.steps, timed, args.oracle_voxels)
     if not args.no_cpu:
         log("CPU baseline (FP64 oracle on host cores)...")
-        cb = cpu_baseline(raw, params, natoms, T, w)
+        cb = cpu_baseline(raw, params, natoms, T, w, cb_rows, cb_gpu, cb_slice, cb_sig, cb_idx)
         out["cpu_baseline"] = cb
-        log(f"CPU baseline: {cb['value']:.1f} atoms/s, {cb['match_value']:.2f} voxels/s "
-            f"on {cb['cores']} threads")
+        log(f"CPU baseline: {cb['value']:.1f} atoms/s, {cb.get('match_value', 0):.2f} voxels/s "
+            f"on {cb['cores']} threads; GPU rows vs FP64 oracle: max rel err "
+            f"{cb['gpu_atoms_max_rel_err']:.2e} on {cb['gpu_atoms_checked']} atoms")
+        out["gen_parity"] = {"atoms_checked": cb["gpu_atoms_checked"],
+                             "max_rel_err_vs_fp64_oracle": cb["gpu_atoms_max_rel_err"],
+                             "tolerance": 1e-5}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -118,7 +118,13 @@ typedef struct {
                              isochromat q of n precesses at
                              B0 + spread*((q + 1/2)/n - 1/2) Hz; the signal
                              is their mean */
-    int reserved[8];
+    int kernel_form;      /* 0: fused frame-form EPG kernel for epg_ssfp/epg_fisp;
+                             1: the reference's unfused step order (cross-check) */
+    int lanes_per_atom;   /* EPG lanes per atom, 1, 2, 4, 8 or 16; 0 = chosen from the
+                             dictionary size.  The choice only changes FP32 rounding
+                             (every form computes the same recursion); it is exposed so
+                             that every lane form can be tested at any size */
+    int reserved[6];
 } mrf_dict_opts;
 
 /* Extended dictionary generation on a caller stream (hipStream_t passed as
@@ -157,6 +163,10 @@ int mrf_match(float *d_maps, const mrf_complex *h_img,
 #define MRF_MATCH_BF16_TOP64 6  /* BF16 MFMA + 64 candidates per chunk + top-64 + rescore: for
                                    dense grids (B0 steps of 2 Hz), where the FP32 argmax can
                                    rank past 32 in BF16 */
+/* Flag OR-ed into a low-precision mode: the candidates are rescored with FP32
+ * sums instead of FP64 sums (FP64 makes the pick the exact argmax among the
+ * candidates; FP32 keeps the rescore in the matcher's own precision). */
+#define MRF_MATCH_RESCORE_FP32 0x100
 
 /* Extended matching, all device-resident, on a caller stream.
  *   d_atoms   [NATOMS][T] complex64 (used by FP32 and for the rescore)

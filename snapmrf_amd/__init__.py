@@ -24,6 +24,7 @@ MRF_MATCH_BF16_TOP16 = 3
 MRF_MATCH_SPLIT2 = 4
 MRF_MATCH_BF16_TOP32 = 5
 MRF_MATCH_BF16_TOP64 = 6
+MRF_MATCH_RESCORE_FP32 = 0x100   # OR into a low-precision mode: FP32 rescore sums
 
 ECHO_TYPES = ("epg_ssfp", "roa_ssfp", "epg_sr", "epg_se", "epg_tse1", "epg_tse2", "epg_fisp")
 
@@ -53,7 +54,8 @@ class MrfRa(ctypes.Structure):
 
 class MrfDictOpts(ctypes.Structure):
     _fields_ = [("normalize", ctypes.c_int), ("n_iso", ctypes.c_int),
-                ("iso_spread_hz", ctypes.c_float), ("reserved", ctypes.c_int * 8)]
+                ("iso_spread_hz", ctypes.c_float), ("kernel_form", ctypes.c_int),
+                ("lanes_per_atom", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
 _lib = None
@@ -258,15 +260,18 @@ def MRF_dict(d_atoms, d_params, h_mrf, nreps: int, NREPS: int, NATOMS: int, nato
 
 def dict_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int, nstates: int,
             d_atoms, d_norm2=None, normalize: int = -1, n_iso: int = 1, iso_spread_hz: float = 0.0,
-            generic: bool = False, stream=None) -> None:
-    """Extended dictionary generation on a stream (asynchronous)."""
+            generic: bool = False, lanes: int = 0, stream=None) -> None:
+    """Extended dictionary generation on a stream (asynchronous).  generic:
+    the unfused reference-order kernel; lanes: EPG lanes per atom (1, 2, 4,
+    8, 16; 0 = chosen from the dictionary size)."""
     import numpy as np
     h = np.ascontiguousarray(h_mrf, dtype=np.float32)
     opts = MrfDictOpts()
     opts.normalize = normalize
     opts.n_iso = n_iso
     opts.iso_spread_hz = iso_spread_hz
-    opts.reserved[0] = 1 if generic else 0
+    opts.kernel_form = 1 if generic else 0
+    opts.lanes_per_atom = lanes
     check(lib().mrf_dict_ex(_stream_ptr(stream), _enc(echo_type), h.ctypes.data, nreps, h.shape[1],
                             _ptr(d_params), NATOMS, natoms, nstates, ctypes.byref(opts),
                             _ptr(d_atoms), _ptr(d_norm2)))
@@ -292,16 +297,19 @@ def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_s
 def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int,
                   nstates: int, d_sig, V: int, d_maps, d_b1map=None, d_idx=None,
                   block_atoms: int = 0, normalize: int = -1, n_iso: int = 1,
-                  iso_spread_hz: float = 0.0, mode: int = 0, stream=None) -> None:
+                  iso_spread_hz: float = 0.0, mode: int = 0, lanes: int = 0,
+                  stream=None) -> None:
     """Dictionary generation streamed through the matcher (mode MRF_MATCH_*),
     block_atoms atoms per slice at a time (0: blocks of at most 1 GiB); with
-    FP32 the maps equal dict_ex + match_ex(MRF_MATCH_FP32)."""
+    FP32 the maps equal dict_ex + match_ex(MRF_MATCH_FP32) with the same
+    lanes (0 on both: each block takes the whole dictionary's lane form)."""
     import numpy as np
     h = np.ascontiguousarray(h_mrf, dtype=np.float32)
     opts = MrfDictOpts()
     opts.normalize = normalize
     opts.n_iso = n_iso
     opts.iso_spread_hz = iso_spread_hz
+    opts.lanes_per_atom = lanes
     check(lib().mrf_dict_match_ex(_stream_ptr(stream), _enc(echo_type), h.ctypes.data, nreps,
                                   h.shape[1], _ptr(d_params), NATOMS, natoms, nstates,
                                   ctypes.byref(opts), mode, _ptr(d_sig), V, _ptr(d_b1map),

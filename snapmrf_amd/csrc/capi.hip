@@ -95,10 +95,11 @@ void prepare_sequence(int echo, const float *h_mrf, size_t nreps, size_t NREPS,
 }
 
 // Atoms of `natoms` parameter rows per slice (n_slices slices of params,
-// rows natoms apart) with the per-slice coefficient table `coef`.
+// rows natoms apart) with the per-slice coefficient table `coef`; `lanes`
+// fixes the EPG lane form (0: chosen from the launch size).
 int gen_atoms(int echo, const TrCoef *coef, const float *params, size_t natoms, int n_slices,
               int T, int nstates, const mrf_dict_opts *opts, bool h_real, int normalize,
-              float2 *atoms, double *n2, hipStream_t s)
+              int lanes, float2 *atoms, double *n2, hipStream_t s)
 {
     int rc = 0;
     if (echo == kRoaSsfp) {
@@ -124,7 +125,8 @@ int gen_atoms(int echo, const TrCoef *coef, const float *params, size_t natoms, 
         l.T = T;
         l.w = nstates;
         l.echo = echo;
-        l.generic = opts && opts->reserved[0] == 1;
+        l.generic = opts && opts->kernel_form == 1;
+        l.lanes = lanes;
         l.h_real = h_real && echo == kEpgFisp;
         l.normalize = normalize != 0;
         rc = launch_epg(l, s);
@@ -186,22 +188,35 @@ extern "C" int mrf_dict_ex(void *stream, const char *echo_type, const float *h_m
 
     float *d_seq = nullptr;
     TrCoef *d_coef = nullptr;
-    double *d_n2 = d_norm2;
-    bool own_n2 = false;
-    MRF_HIP(hipMallocAsync((void **)&d_seq, 4 * nreps * sizeof(float), s));
-    MRF_HIP(hipMallocAsync((void **)&d_coef, (size_t)n_slices * T * sizeof(TrCoef), s));
+    double *d_n2 = d_norm2, *own_n2 = nullptr;
+    int rc = 0;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (rc) return;
+        hipError_t e = hipMallocAsync(p, bytes, s);
+        if (e != hipSuccess)
+            rc = mrf_fail(MRF_E_NOMEM, "device allocation of %zu bytes: %s", bytes,
+                          hipGetErrorString(e));
+    };
+    alloc((void **)&d_seq, 4 * nreps * sizeof(float));
+    alloc((void **)&d_coef, (size_t)n_slices * T * sizeof(TrCoef));
     if (!d_n2 && normalize) {
-        MRF_HIP(hipMallocAsync((void **)&d_n2, n_gen * sizeof(double), s));
-        own_n2 = true;
+        alloc((void **)&own_n2, n_gen * sizeof(double));
+        d_n2 = own_n2;
     }
-    MRF_HIP(hipMemcpyAsync(d_seq, seq.data(), 4 * nreps * sizeof(float), hipMemcpyHostToDevice, s));
-    coef_kernel<<<dim3((T + 127) / 128), 128, 0, s>>>(d_seq, T, d_params, natoms, n_slices,
-                                                       echo == kEpgSsfp ? 1 : 0, d_coef);
-    int rc = gen_atoms(echo, d_coef, d_params, natoms, n_slices, T, nstates, opts, h_real,
-                       normalize, reinterpret_cast<float2 *>(d_atoms), d_n2, s);
-    (void)hipFreeAsync(d_seq, s);
-    (void)hipFreeAsync(d_coef, s);
-    if (own_n2) (void)hipFreeAsync(d_n2, s);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(d_seq, seq.data(), 4 * nreps * sizeof(float),
+                                      hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = mrf_fail(MRF_E_GPU, "sequence upload: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        coef_kernel<<<dim3((T + 127) / 128), 128, 0, s>>>(d_seq, T, d_params, natoms, n_slices,
+                                                           echo == kEpgSsfp ? 1 : 0, d_coef);
+        rc = gen_atoms(echo, d_coef, d_params, natoms, n_slices, T, nstates, opts, h_real,
+                       normalize, opts ? opts->lanes_per_atom : 0,
+                       reinterpret_cast<float2 *>(d_atoms), d_n2, s);
+    }
+    for (void *p : {(void *)d_seq, (void *)d_coef, (void *)own_n2})
+        if (p) (void)hipFreeAsync(p, s);
     return rc;
 }
 
@@ -224,6 +239,10 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                             size_t V, const float *d_b1map, float *d_maps, int32_t *d_idx)
 {
     mrf_clear_error();
+    const bool rescore_fp32 = (mode & MRF_MATCH_RESCORE_FP32) != 0;
+    mode &= ~MRF_MATCH_RESCORE_FP32;
+    if (rescore_fp32 && mode == MRF_MATCH_FP32)
+        return mrf_fail(MRF_E_ARG, "the FP32 rescore flag applies to low-precision modes");
     if (!d_atoms || !d_params || (V && (!d_sig || !d_maps)))
         return mrf_fail(MRF_E_ARG, "null pointer argument");
     if (T == 0 || T > 0x7fffffff) return mrf_fail(MRF_E_ARG, "bad T");
@@ -238,30 +257,39 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
     if (V == 0) return MRF_OK;
     hipStream_t s = (hipStream_t)stream;
     const int n_slices = (int)(NATOMS / natoms);
-    int32_t *d_vs = nullptr;
-    if (d_b1map) {
-        MRF_HIP(hipMallocAsync((void **)&d_vs, V * sizeof(int32_t), s));
-        if (int rc = launch_b1_select(d_b1map, d_params, natoms, n_slices, V, d_vs, s)) return rc;
-    }
     MatchLaunch l{};
     l.atoms = reinterpret_cast<const float2 *>(d_atoms);
     l.lowp_hi = (const uint32_t *)d_lowp;
     l.lowp_lo = (const uint32_t *)d_lowp_lo;
     l.params = d_params;
     l.sig = reinterpret_cast<const float2 *>(d_sig);
-    l.vox_slice = d_vs;
     l.natoms = natoms;
     l.n_slices = n_slices;
     l.T = (int)T;
     l.V = V;
     l.mode = mode;
+    l.rescore_fp32 = rescore_fp32;
     l.maps = d_maps;
     l.idx = d_idx;
+    int32_t *d_vs = nullptr;
     void *scratch = nullptr;
-    MRF_HIP(hipMallocAsync(&scratch, match_scratch_bytes(l), s));
-    int rc = launch_match(l, scratch, s);
-    (void)hipFreeAsync(scratch, s);
-    if (d_vs) (void)hipFreeAsync(d_vs, s);
+    int rc = 0;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (rc) return;
+        hipError_t e = hipMallocAsync(p, bytes, s);
+        if (e != hipSuccess)
+            rc = mrf_fail(MRF_E_NOMEM, "device allocation of %zu bytes: %s", bytes,
+                          hipGetErrorString(e));
+    };
+    if (d_b1map) alloc((void **)&d_vs, V * sizeof(int32_t));
+    if (!rc) {
+        l.vox_slice = d_vs;   // scratch size depends on the slice bucketing
+        alloc(&scratch, match_scratch_bytes(l));
+    }
+    if (!rc && d_b1map) rc = launch_b1_select(d_b1map, d_params, natoms, n_slices, V, d_vs, s);
+    if (!rc) rc = launch_match(l, scratch, s);
+    for (void *p : {(void *)d_vs, scratch})
+        if (p) (void)hipFreeAsync(p, s);
     return rc;
 }
 
@@ -273,6 +301,10 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
                                  float *d_maps, int32_t *d_idx)
 {
     mrf_clear_error();
+    const bool rescore_fp32 = (mode & MRF_MATCH_RESCORE_FP32) != 0;
+    mode &= ~MRF_MATCH_RESCORE_FP32;
+    if (rescore_fp32 && mode == MRF_MATCH_FP32)
+        return mrf_fail(MRF_E_ARG, "the FP32 rescore flag applies to low-precision modes");
     const int echo = echo_id(echo_type);
     if (echo < 0)
         return mrf_fail(MRF_E_ECHO, "unknown echo type '%s'", echo_type ? echo_type : "(null)");
@@ -307,6 +339,20 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     bool h_real = true;
     for (size_t i = 1; i < nreps; i++)
         if (seq[nreps + i] != seq[nreps]) h_real = false;
+    // Every block is generated in the lane form mrf_dict_ex picks for the
+    // whole dictionary: the form sets the FP32 rounding of each atom, so the
+    // blocks then hold exactly the rows the whole dictionary would.
+    int lanes = opts ? opts->lanes_per_atom : 0;
+    if (echo != kRoaSsfp && lanes == 0) {
+        EpgLaunch whole{};
+        whole.natoms = natoms;
+        whole.n_slices = n_slices;
+        whole.w = nstates;
+        whole.echo = echo;
+        whole.generic = opts && opts->kernel_form == 1;
+        int M = 0;
+        if (int rc = epg_form(whole, &lanes, &M)) return rc;
+    }
 
     MatchLaunch l{};
     l.params = d_params;
@@ -316,6 +362,7 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
     l.T = T;
     l.V = V;
     l.mode = mode;
+    l.rescore_fp32 = rescore_fp32;
     l.maps = d_maps;
     l.idx = d_idx;
     if (mode < MRF_MATCH_FP32 || mode > MRF_MATCH_BF16_TOP64)
@@ -374,7 +421,7 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
             if (e != hipSuccess)
                 return mrf_fail(MRF_E_GPU, "parameter gather: %s", hipGetErrorString(e));
             return gen_atoms(echo, d_coef, d_pblk, nb, n_slices, T, nstates, opts, h_real,
-                             normalize, d_buf, d_n2, s);
+                             normalize, lanes, d_buf, d_n2, s);
         };
         rc = launch_match_blocks(l, block, d_buf, mode != MRF_MATCH_FP32 ? &lb : nullptr, scratch, s,
                                  gen);

@@ -909,15 +909,26 @@ __global__ __launch_bounds__(kBlock) void gn_trial_kernel(EpgArgs a, const float
 
 }  // namespace
 
-int launch_epg(const EpgLaunch &l, hipStream_t s)
+int epg_form(const EpgLaunch &l, int *L_out, int *M_out)
 {
     if (l.w < 2) return mrf_fail(MRF_E_ARG, "nstates must be >= 2 (got %d)", l.w);
     const bool fused = !l.generic && (l.echo == kEpgSsfp || l.echo == kEpgFisp);
-    // choose lanes per atom L and states per lane M
-    // (fused: smallest L, then smallest compiled M with L*M >= w; reference-
-    // order path: M = 8 up to w = 128, else M = 32)
     int L = 1, M = 0;
-    if (fused) {
+    if (l.lanes) {
+        // requested lanes per atom: the smallest compiled M that covers w
+        if (l.lanes != 1 && l.lanes != 2 && l.lanes != 4 && l.lanes != 8 && l.lanes != 16)
+            return mrf_fail(MRF_E_ARG, "lanes per atom must be 1, 2, 4, 8 or 16 (got %d)", l.lanes);
+        L = l.lanes;
+        if (fused) {
+            for (int m : kMs)
+                if (m * L >= l.w) { M = m; break; }
+        } else {
+            M = (8 * L >= l.w) ? 8 : ((32 * L >= l.w) ? 32 : 0);
+        }
+        if (!M)
+            return mrf_fail(MRF_E_UNSUPPORTED, "%d lanes per atom cannot hold %d states", L, l.w);
+    } else if (fused) {
+        // smallest L, then smallest compiled M with L*M >= w
         for (L = 1; L <= 16 && !M; L *= 2)
             for (int m : kMs)
                 if (m * L >= l.w) { M = m; break; }
@@ -934,11 +945,22 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
             M = m2;
         }
     } else {
+        // reference-order path: M = 8 up to w = 128, else M = 32
         const int mw = (l.w <= 128) ? 8 : 32;
         for (L = 1; L <= 16 && mw * L < l.w; L *= 2) {}
         if (L <= 16) M = mw;
     }
     if (!M || L > 16) return mrf_fail(MRF_E_UNSUPPORTED, "nstates %d exceeds 512", l.w);
+    *L_out = L;
+    *M_out = M;
+    return 0;
+}
+
+int launch_epg(const EpgLaunch &l, hipStream_t s)
+{
+    const bool fused = !l.generic && (l.echo == kEpgSsfp || l.echo == kEpgFisp);
+    int L = 1, M = 0;
+    if (int rc = epg_form(l, &L, &M)) return rc;
 
     EpgArgs args{};
     args.coef = l.coef;

@@ -347,6 +347,21 @@ extern "C" int mrf_ra_read(mrf_ra_t *a, const char *path)
         mrf_ra_free(a);
         return mrf_fail(MRF_E_FORMAT, "%s: truncated RA dims", path);
     }
+    // size must be elbyte * prod(dims) (src/ra.h:20-58), computed without
+    // overflow: callers size their copies from the dims
+    uint64_t want = a->elbyte;
+    bool overflow = false;
+    for (uint64_t d = 0; d < a->ndims; d++) {
+        if (a->dims[d] && want > UINT64_MAX / a->dims[d]) overflow = true;
+        want *= a->dims[d];
+    }
+    if (overflow || want != a->size) {
+        ::close(fd);
+        const unsigned long long sz = (unsigned long long)a->size;
+        mrf_ra_free(a);
+        return mrf_fail(MRF_E_FORMAT, "%s: header size %llu is not elbyte x prod(dims)%s", path,
+                        sz, overflow ? " (dims overflow)" : "");
+    }
     a->data = (uint8_t *)std::malloc(a->size ? a->size : 1);
     if (!a->data) {
         ::close(fd);

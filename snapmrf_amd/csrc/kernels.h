@@ -26,7 +26,15 @@ struct EpgLaunch {
     bool generic;          // force the unfused reference-order path
     bool h_real;           // constant RF phase and no B0 rotation (FISP)
     bool normalize;        // unit L2 norm per atom (norm2 must be non-null)
+    int lanes;             // lanes per atom (1, 2, 4, 8, 16); 0 = chosen from the launch size
 };
+
+// Lane form of a launch: lanes per atom *L and states per lane *M.  With
+// lanes == 0 the fused path takes the fewest lanes whose compiled M covers
+// w, then spreads launches of fewer than 2^17 lanes over more lanes per
+// atom (at least 8 states each).  Two launches of the same form compute
+// every atom with the same FP32 operations.
+int epg_form(const EpgLaunch &a, int *L, int *M);
 
 // EPG dictionary: SSFP/FISP fused path or SR/SE/TSE reference-order path.
 int launch_epg(const EpgLaunch &a, hipStream_t s);
@@ -59,7 +67,8 @@ struct MatchLaunch {
     int n_slices;
     int T;
     size_t V;
-    int mode;                   // MRF_MATCH_*
+    int mode;                   // MRF_MATCH_* (without MRF_MATCH_RESCORE_FP32)
+    bool rescore_fp32;          // low-precision modes: rescore sums in FP32 (default FP64)
     float *maps;                // [V][5]
     int32_t *idx;               // [V] or nullptr
 };

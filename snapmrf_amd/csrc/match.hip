@@ -845,7 +845,10 @@ __device__ __forceinline__ size_t atom_row(const RescoreArgs &r, uint32_t gi)
     return k * r.nb + (gi - k * r.gnat - r.a0);
 }
 
-template <int K, int CK>
+// Acc: double (default; products of two floats are exact, so the pick is the
+// exact argmax among the candidates) or float (sums in FP32, the order below:
+// per lane over t = lane + 64 i, then a butterfly over the 64 lanes).
+template <int K, int CK, typename Acc>
 __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(RescoreArgs ra)
 {
     const float *part_s = ra.part_s;
@@ -925,13 +928,13 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
     // row and are ignored), so the loop body has no branches and a group's
     // row loads are all in flight together.
     constexpr int G = K < 16 ? K : 16;
-    double bs = -1.0;
+    Acc bs = -1.0f;
     uint32_t bi = 0u;
     const float2 *x = sig + vox * (size_t)T;
 #pragma unroll
     for (int c0 = 0; c0 < K; c0 += G) {
         if (c0 >= nc) break;
-        double re[G], im[G];
+        Acc re[G], im[G];
         const float2 *row[G];
 #pragma unroll
         for (int c = 0; c < G; c++) {
@@ -951,10 +954,10 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
                 for (int c = 0; c < G; c++) av[c] = reinterpret_cast<const float4 *>(row[c])[t];
 #pragma unroll
                 for (int c = 0; c < G; c++) {
-                    re[c] += (double)av[c].x * xv.x + (double)av[c].y * xv.y +
-                             ((double)av[c].z * xv.z + (double)av[c].w * xv.w);
-                    im[c] += (double)av[c].x * xv.y - (double)av[c].y * xv.x +
-                             ((double)av[c].z * xv.w - (double)av[c].w * xv.z);
+                    re[c] += (Acc)av[c].x * xv.x + (Acc)av[c].y * xv.y +
+                             ((Acc)av[c].z * xv.z + (Acc)av[c].w * xv.w);
+                    im[c] += (Acc)av[c].x * xv.y - (Acc)av[c].y * xv.x +
+                             ((Acc)av[c].z * xv.w - (Acc)av[c].w * xv.z);
                 }
             }
         } else {
@@ -965,8 +968,8 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
                 for (int c = 0; c < G; c++) av[c] = row[c][t];
 #pragma unroll
                 for (int c = 0; c < G; c++) {
-                    re[c] += (double)av[c].x * xv.x + (double)av[c].y * xv.y;
-                    im[c] += (double)av[c].x * xv.y - (double)av[c].y * xv.x;
+                    re[c] += (Acc)av[c].x * xv.x + (Acc)av[c].y * xv.y;
+                    im[c] += (Acc)av[c].x * xv.y - (Acc)av[c].y * xv.x;
                 }
             }
         }
@@ -979,7 +982,7 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
 #pragma unroll
         for (int c = 0; c < G; c++)
             if (c0 + c < nc) {
-                const double sc = re[c] * re[c] + im[c] * im[c];
+                const Acc sc = re[c] * re[c] + im[c] * im[c];
                 uint32_t ci;
                 if constexpr (kCandLds) ci = cand_lds[threadIdx.x >> 6][c0 + c];
                 else ci = cand[c0 + c];
@@ -989,23 +992,24 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
                 }
             }
     }
+    double bsd = (double)bs;   // an FP32 score is exact in double
     if (nc == 0) {
-        bs = -1.0;
+        bsd = -1.0;
         bi = 0xffffffffu;
     }
     if (lane == 0) {
         if (ra.best_s) {
             const double ps = ra.best_s[vox];
             const uint32_t pi = ra.best_i[vox];
-            if (!(bs > ps || (bs == ps && bi < pi))) {
-                bs = ps;
+            if (!(bsd > ps || (bsd == ps && bi < pi))) {
+                bsd = ps;
                 bi = pi;
             }
-            ra.best_s[vox] = bs;
+            ra.best_s[vox] = bsd;
             ra.best_i[vox] = bi;
         }
-        if (bs < 0.0) {   // no candidate: the reference leaves index 0
-            bs = 0.0;
+        if (bsd < 0.0) {   // no candidate: the reference leaves index 0
+            bsd = 0.0;
             bi = 0u;
         }
         const float *p = params + (size_t)bi * 4;
@@ -1013,7 +1017,7 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
         ra.maps[vox * 5 + 1] = p[1];
         ra.maps[vox * 5 + 2] = p[2];
         ra.maps[vox * 5 + 3] = p[3];
-        ra.maps[vox * 5 + 4] = (float)sqrt(bs);
+        ra.maps[vox * 5 + 4] = (float)sqrt(bsd);
         if (ra.idx) ra.idx[vox] = (int32_t)bi;
     }
 }
@@ -1278,28 +1282,34 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     // the registers of two waves per SIMD).
     const dim3 rg((unsigned)((l.V + 3) / 4));
     const dim3 blk(kLowpThreads);
+#define MRF_RESCORE(K, CK)                                                  \
+    do {                                                                    \
+        if (l.rescore_fp32) rescore_kernel<K, CK, float><<<rg, kBlock, 0, s>>>(r);   \
+        else rescore_kernel<K, CK, double><<<rg, kBlock, 0, s>>>(r);                 \
+    } while (0)
     if (l.mode == MRF_MATCH_SPLIT3) {
         b.nk = (l.T + 15) / 16;
         match_lowp_kernel<3, 16, 2, 2><<<grid, blk, 0, s>>>(b);
-        rescore_kernel<8, 16><<<rg, kBlock, 0, s>>>(r);
+        MRF_RESCORE(8, 16);
     } else if (l.mode == MRF_MATCH_SPLIT2) {
         b.nk = (l.T + 15) / 16;
         match_lowp_kernel<2, 16, 4, 2><<<grid, blk, 0, s>>>(b);
-        rescore_kernel<8, 32><<<rg, kBlock, 0, s>>>(r);
+        MRF_RESCORE(8, 32);
     } else if (l.mode == MRF_MATCH_BF16_TOP64) {
         b.nk = (l.T + 15) / 16;
         match_lowp_kernel<1, 16, 8, 1><<<grid, blk, 0, s>>>(b);
-        rescore_kernel<64, 64><<<rg, kBlock, 0, s>>>(r);
+        MRF_RESCORE(64, 64);
     } else if (l.mode == MRF_MATCH_BF16_TOP32) {
         b.nk = (l.T + 31) / 32;
         match_lowp_kernel<1, 32, 6, 1><<<grid, blk, 0, s>>>(b);
-        rescore_kernel<32, 48><<<rg, kBlock, 0, s>>>(r);
+        MRF_RESCORE(32, 48);
     } else {
         b.nk = (l.T + 31) / 32;
         match_lowp_kernel<1, 32, 3, 2><<<grid, blk, 0, s>>>(b);
-        if (l.mode == MRF_MATCH_BF16_TOP16) rescore_kernel<16, 24><<<rg, kBlock, 0, s>>>(r);
-        else rescore_kernel<8, 24><<<rg, kBlock, 0, s>>>(r);
+        if (l.mode == MRF_MATCH_BF16_TOP16) MRF_RESCORE(16, 24);
+        else MRF_RESCORE(8, 24);
     }
+#undef MRF_RESCORE
     return 0;
 }
 

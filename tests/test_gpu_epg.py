@@ -74,8 +74,12 @@ def test_fisp_config1_vs_oracle(gpu):
 
 @pytest.mark.parametrize("w", [30, 60, 101, 200])
 def test_ssfp_b0_b1_grid_vs_oracle(gpu, w):
-    """B1-sliced SSFP with off-resonance; w = 30 (1 lane/atom), 60 (2),
-    101 (4, padded), 200 (8, padded)."""
+    """B1-sliced SSFP with off-resonance on 1,530 atoms.  A launch this small
+    is spread over more lanes per atom (until it has 2^17 lanes, at least 8
+    states per lane): w = 30 runs 4 lanes of 8 states, 60 runs 8 x 8, 101
+    and 200 run 16 x 8 and 16 x 16 (padded, truncated at w - 1).  The
+    one-lane form is tested by test_lane_forms_vs_oracle and at full size by
+    test_full_grid_one_lane_form_vs_oracle."""
     T = 400 if w > 60 else 1000
     s = synth.sequence(T, seed=2)
     raw = synth.csv_columns(s, "epg_ssfp")
@@ -152,4 +156,97 @@ def test_truncated_and_odd_lengths(gpu, echo, w, T):
     params, A, natoms = sm.build_params("150:350:2000", "25:45:300", "-35:35:35", "1")
     got = gen(gpu, echo, raw, params, natoms, w)
     ref = oracle.epg(echo, oracle.seq_prepare(echo, raw), params, natoms, w)
+    assert rel_err(got, ref) < TOL
+
+
+# 1,530 atoms in 3 B1 slices of 510: the last wave of each slice holds 62
+# atoms, so the one-lane form's staged writer and in-kernel normalisation run
+# a partial wave
+LANE_GRID = ("100:300:2000+2300:900:5000", "20:40:300+400:800:2000", "-40:20:40", "0.8:0.2:1.2")
+
+
+@pytest.mark.parametrize("echo,w,T,lanes,normalize", [
+    ("epg_ssfp", 30, 1000, 1, -1),   # the full-size form: M = 30, in-kernel normalisation
+    ("epg_ssfp", 30, 333, 1, 0),     # odd T, unnormalised (norms returned)
+    ("epg_ssfp", 20, 999, 1, 1),     # M = 20, odd T
+    ("epg_ssfp", 20, 1000, 1, 0),
+    ("epg_ssfp", 17, 64, 1, -1),     # M = 20 padded: truncation at w - 1 in the one-lane form
+    ("epg_ssfp", 7, 30, 1, -1),      # T below two 16-TR output segments
+    ("epg_ssfp", 30, 5, 1, -1),      # T shorter than one segment
+    ("epg_fisp", 20, 1000, 1, 0),    # constant-phase FISP branch
+    ("epg_ssfp", 30, 500, 2, 1),     # lane pairs of 16 (truncated), separate normalisation
+    ("epg_ssfp", 30, 400, 16, 1),    # 16 lanes of 4 states (64, truncated at w - 1)
+])
+def test_lane_forms_vs_oracle(gpu, echo, w, T, lanes, normalize):
+    """Each lane form of the fused kernel, forced through
+    mrf_dict_opts.lanes_per_atom, against the FP64 oracle (1e-5 of each
+    atom's infinity norm); normalised atoms have unit norm, and the norms
+    returned in d_norm2 are those of the unnormalised rows."""
+    s = synth.sequence(T, seed=w + T)
+    raw = synth.csv_columns(s, echo)
+    params, A, natoms = sm.build_params(*LANE_GRID)
+    if echo != "epg_ssfp":
+        params, A = params[:natoms], natoms
+    d_params = gpu.from_numpy(params).cuda()
+    d_atoms = gpu.zeros((A, T), dtype=gpu.complex64, device="cuda")
+    d_n2 = gpu.zeros(A, dtype=gpu.float64, device="cuda")
+    sm.dict_ex(echo, raw, T, d_params, A, natoms, w, d_atoms, d_norm2=d_n2, normalize=normalize,
+               lanes=lanes)
+    gpu.cuda.synchronize()
+    got = d_atoms.cpu().numpy().astype(np.complex128)
+    n2 = d_n2.cpu().numpy()
+    ref = oracle.epg(echo, oracle.seq_prepare(echo, raw), params, natoms, w)
+    norms = np.linalg.norm(got, axis=1)
+    if echo == "epg_ssfp" and normalize != 0:
+        assert np.allclose(norms, 1.0, atol=1e-5)
+        assert rel_err(got, ref) < TOL
+    else:
+        assert np.allclose(n2, norms ** 2, rtol=1e-5)
+        unit = got / norms[:, None] if echo == "epg_ssfp" else got
+        assert rel_err(unit, ref) < TOL
+
+
+def test_lanes_per_atom_rejects_bad_values(gpu):
+    raw = synth.csv_columns(synth.sequence(20, seed=0), "epg_ssfp")
+    params, A, natoms = sm.build_params(*LANE_GRID)
+    d_params = gpu.from_numpy(params).cuda()
+    d_atoms = gpu.zeros((A, 20), dtype=gpu.complex64, device="cuda")
+    for lanes, w in ((3, 30), (32, 30), (1, 200)):   # not a power of two, > 16, M > 32 at L = 1
+        with pytest.raises(sm.MRFError):
+            sm.dict_ex("epg_ssfp", raw, 20, d_params, A, natoms, w, d_atoms, lanes=lanes)
+
+
+C2_GRID = ("100:20:2000+2300:300:5000", "20:5:100+110:10:300+400:200:2000", "-40:2:40",
+           "0.8:0.1:1.2")
+
+
+@pytest.mark.parametrize("w", [30, 20])
+def test_full_grid_one_lane_form_vs_oracle(gpu, w):
+    """The 871,455-atom SSFP grid (5 B1 slices of 174,291, B0 -40:2:40 Hz),
+    T = 1000: a launch this size runs one atom per lane with the LDS-staged
+    row writer and in-kernel normalisation.  It equals the forced one-lane
+    launch bit for bit; every atom has unit norm; a seeded subset of 2,100
+    atoms (420 grid points in each of the 5 slices) equals the FP64 oracle
+    within 1e-5 of each atom's infinity norm."""
+    import torch
+    T = 1000
+    raw = synth.csv_columns(synth.sequence(T, seed=0), "epg_ssfp")
+    params, A, natoms = sm.build_params(*C2_GRID)
+    assert (A, natoms) == (871455, 174291)
+    d_params = torch.from_numpy(params).cuda()
+    d_atoms = torch.empty((A, T), dtype=torch.complex64, device="cuda")
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_atoms)
+    d_one = torch.empty_like(d_atoms)
+    sm.dict_ex("epg_ssfp", raw, T, d_params, A, natoms, w, d_one, lanes=1)
+    torch.cuda.synchronize()
+    assert torch.equal(d_atoms, d_one)
+    del d_one
+    norms = torch.linalg.vector_norm(d_atoms, dim=1)
+    assert float((norms - 1.0).abs().max()) < 1e-5
+    rng = np.random.default_rng(100 + w)
+    pick = np.sort(rng.choice(natoms, 420, replace=False))
+    rows = (np.arange(5)[:, None] * natoms + pick[None, :]).ravel()
+    got = d_atoms[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.complex128)
+    sub = params[rows]
+    ref = oracle.epg("epg_ssfp", oracle.seq_prepare("epg_ssfp", raw), sub, len(pick), w)
     assert rel_err(got, ref) < TOL

@@ -1,7 +1,8 @@
 """Dictionary generation streamed through the matcher (mrf_dict_match_ex): the
 dictionary exists one block of atoms per slice at a time.  FP32: the maps and
 indices must equal generating the whole dictionary (mrf_dict_ex) and matching
-it (mrf_match_ex) bit for bit, for any block size, with and without a B1 map.
+it (mrf_match_ex) bit for bit, for any block size, with and without a B1 map;
+every block is generated in the lane form of the whole dictionary.
 Low-precision modes: the streamed pick must score at least as high in FP64 as
 the whole-dictionary call's.  The whole-dictionary path is pinned to the
 reference's fixtures and the FP64 oracle by test_gpu_epg.py and
@@ -192,3 +193,59 @@ def _streamed_generic(torch, raw, T_, d_params, A, natoms, sig, d_b1, block, mod
                      d_idx=d_idx, block_atoms=block, mode=mode)
     torch.cuda.synchronize()
     return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+
+
+def test_streamed_blocks_keep_the_whole_dictionarys_lane_form(gpu):
+    """w = 30, T = 200 on 267,813 atoms (3 B1 slices of 89,271): the whole
+    dictionary has more than 2^17 atoms and runs one atom per lane, blocks of
+    50,000 atoms per slice leave a last block of 39,271 x 3 = 117,813 atoms,
+    which on its own would be spread over lane pairs.  The blocks must take
+    the whole dictionary's form, so the maps -- proton density included,
+    which moves with any rounding difference in the atoms -- equal the whole
+    dictionary's bit for bit.  The whole dictionary generated in the two-lane
+    form instead gives different proton densities, which shows the comparison
+    can see a change of form."""
+    import torch
+    Ts, Ws = 200, 30
+    grid = ("100:20:2000+2300:300:5000", "20:5:100+110:10:300+400:200:2000", "-40:4:40",
+            "0.8:0.2:1.2")
+    raw = synth.csv_columns(synth.sequence(Ts, seed=3), "epg_ssfp")
+    params, A, natoms = sm.build_params(*grid)
+    assert (A, natoms) == (267813, 89271)
+    d_params = torch.from_numpy(params).cuda()
+    V = 4096
+    rng = np.random.default_rng(33)
+    slice_b1 = params[::natoms, 3]
+    b1 = rng.uniform(0.75, 1.25, V).astype(np.float32)
+    ks = np.argmin(np.abs(b1[:, None] - slice_b1[None, :]), axis=1)
+    truth = ks * natoms + rng.integers(0, natoms, V)
+    d_b1 = torch.from_numpy(b1).cuda()
+
+    def whole(lanes):
+        d_atoms = torch.empty((A, Ts), dtype=torch.complex64, device="cuda")
+        sm.dict_ex("epg_ssfp", raw, Ts, d_params, A, natoms, Ws, d_atoms, lanes=lanes)
+        return d_atoms
+    d_atoms = whole(0)
+    sig = d_atoms[torch.from_numpy(truth).cuda()] * 0.8
+    g = torch.Generator(device="cuda").manual_seed(34)
+    sig = (sig + 0.01 * torch.complex(torch.randn(V, Ts, device="cuda", generator=g),
+                                      torch.randn(V, Ts, device="cuda", generator=g))).contiguous()
+
+    def match_whole(atoms):
+        d_maps = torch.empty((V, 5), device="cuda")
+        d_idx = torch.empty(V, dtype=torch.int32, device="cuda")
+        sm.match_ex(sm.MRF_MATCH_FP32, atoms, d_params, Ts, A, natoms, sig, V, d_maps,
+                    d_b1map=d_b1, d_idx=d_idx)
+        torch.cuda.synchronize()
+        return d_maps.cpu().numpy(), d_idx.cpu().numpy()
+    m_ref, i_ref = match_whole(d_atoms)
+    d_maps = torch.full((V, 5), -7.0, device="cuda")
+    d_idx = torch.full((V,), -7, dtype=torch.int32, device="cuda")
+    sm.dict_match_ex("epg_ssfp", raw, Ts, d_params, A, natoms, Ws, sig, V, d_maps, d_b1map=d_b1,
+                     d_idx=d_idx, block_atoms=50000)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_idx.cpu().numpy(), i_ref)
+    assert np.array_equal(d_maps.cpu().numpy(), m_ref)
+    del d_atoms
+    m2, _ = match_whole(whole(2))
+    assert not np.array_equal(m2[:, 4], m_ref[:, 4])

@@ -3,6 +3,7 @@ parser (src/functions.cu:196-414), rfpulse CSV reader (:100-158), RA I/O
 (src/ra.cu) and the mrf CLI's argument handling.  No GPU calls."""
 import os
 import re
+import struct
 import subprocess
 import sys
 
@@ -165,6 +166,31 @@ def test_ra_rejects_bad_files(tmp_path):
     open(q, "wb").write(data[:-8])          # truncated data
     with pytest.raises(sm.MRFError):
         sm.ra_read(q)
+
+
+def _ra_bytes(eltype, elbyte, size, dims, payload):
+    return struct.pack("<6Q", ra_util.MAGIC, 0, eltype, elbyte, size, len(dims)) + \
+        struct.pack("<%dQ" % len(dims), *dims) + payload
+
+
+@pytest.mark.parametrize("size,dims", [
+    (16, [4, 4]),                       # header size smaller than its dims imply
+    (80, [4, 4]),                       # larger
+    (8, [1 << 62, 8]),                  # dims product overflows 64 bits
+    (0, [0, 5]),                        # zero-sized: fine (reads nothing)
+])
+def test_ra_header_size_must_match_dims(tmp_path, size, dims):
+    """The readers of images and B1 maps size their copies from dims, so a
+    header whose size disagrees with elbyte * prod(dims) is rejected (the
+    reference's ra_read trusts both, src/ra.cu:23-55)."""
+    p = str(tmp_path / "h.ra")
+    open(p, "wb").write(_ra_bytes(3, 4, size, dims, b"\0" * size))
+    if size == 4 * dims[0] * dims[1] and size < (1 << 40):
+        arr, hdr = sm.ra_read(p)
+        assert hdr["size"] == size
+    else:
+        with pytest.raises(sm.MRFError, match="prod"):
+            sm.ra_read(p)
 
 
 def test_cli_usage_and_errors(tmp_path):
