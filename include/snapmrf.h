@@ -187,6 +187,23 @@ int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
                  size_t natoms, const mrf_complex *d_sig, size_t V,
                  const float *d_b1map, float *d_maps, int32_t *d_idx);
 
+/* FP32 matching (mode MRF_MATCH_FP32 of mrf_match_ex: same arguments, same
+ * kernels, the same d_maps / d_idx) that allocates no device memory.  No
+ * counterpart in the reference (MRF_match allocates its buffers per call,
+ * src/functions.cu:1493-1584).  For callers that cannot allocate: code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc.  Its
+ * scratch (voxel slices, per-voxel keys, voxel permutation, tile list) is a
+ * static 20 MiB device array, one per GPU, held until the process ends:
+ * calls with more voxels than fit (2^20 voxels over up to 1,024 B1 slices
+ * always fit) fail with MRF_E_ARG before any launch.  Every call on a GPU
+ * uses that GPU's one array, so calls that overlap on one GPU (several
+ * streams or host threads) race: serialise them.  The scratch is cleared by
+ * kernels on the stream at each call, so consecutive calls are independent. */
+int mrf_match_ex_noalloc(void *stream, const mrf_complex *d_atoms,
+                         const float *d_params, size_t T, size_t NATOMS,
+                         size_t natoms, const mrf_complex *d_sig, size_t V,
+                         const float *d_b1map, float *d_maps, int32_t *d_idx);
+
 /* Dictionary generation streamed through the matcher: the atoms exist one
  * block at a time (block_atoms per B1 slice, every slice; 0 = blocks of at
  * most 1 GiB), each block is generated as mrf_dict_ex would generate it and

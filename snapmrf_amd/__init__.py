@@ -33,7 +33,7 @@ EXPORTS = (
     "mrf_last_error", "mrf_csv_dim", "mrf_csv_read", "mrf_parse_length", "mrf_parse_params",
     "mrf_logspace", "mrf_compute_natoms", "mrf_trans_params", "mrf_grid_natoms_mode",
     "mrf_ra_read", "mrf_ra_write", "mrf_ra_free", "mrf_dict", "mrf_dict_ex", "mrf_match",
-    "mrf_match_ex", "mrf_dict_match_ex", "mrf_svd_basis", "mrf_svd_compress", "mrf_svd_match_ex",
+    "mrf_match_ex", "mrf_match_ex_noalloc", "mrf_dict_match_ex", "mrf_svd_basis", "mrf_svd_compress", "mrf_svd_match_ex",
     "mrf_lowp_prepare", "mrf_lowp_ld", "mrf_epg_jacobian", "mrf_epg_adjoint",
     "mrf_gn_refine", "mrf_crb_loss_grad", "mrf_device_count",
 )
@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
         "mrf_dict_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), P, P], I),
         "mrf_match": ([P, P, P, P, S, S, S, S, S, I, I, cs], I),
         "mrf_match_ex": ([P, I, P, P, P, P, S, S, S, P, S, P, P, P], I),
+        "mrf_match_ex_noalloc": ([P, P, P, S, S, S, P, S, P, P, P], I),
         "mrf_dict_match_ex": ([P, cs, P, S, S, P, S, S, I, ctypes.POINTER(MrfDictOpts), I, P, S,
                                P, S, P, P], I),
         "mrf_svd_basis": ([P, P, S, S, S, I, ctypes.c_double, S, P, P, ctypes.POINTER(S)], I),
@@ -292,6 +293,16 @@ def match_ex(mode: int, d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_s
     check(lib().mrf_match_ex(_stream_ptr(stream), mode, _ptr(d_atoms), _ptr(d_lowp),
                              _ptr(d_lowp_lo), _ptr(d_params), T, NATOMS, natoms, _ptr(d_sig), V,
                              _ptr(d_b1map), _ptr(d_maps), _ptr(d_idx)))
+
+
+def match_ex_noalloc(d_atoms, d_params, T: int, NATOMS: int, natoms: int, d_sig, V: int,
+                     d_maps, d_b1map=None, d_idx=None, stream=None) -> None:
+    """FP32 matching as match_ex(MRF_MATCH_FP32) with no device allocation
+    (static per-GPU scratch: at most 2^20 voxels; overlapping calls on one GPU
+    race)."""
+    check(lib().mrf_match_ex_noalloc(_stream_ptr(stream), _ptr(d_atoms), _ptr(d_params), T,
+                                     NATOMS, natoms, _ptr(d_sig), V, _ptr(d_b1map), _ptr(d_maps),
+                                     _ptr(d_idx)))
 
 
 def dict_match_ex(echo_type: str, h_mrf, nreps: int, d_params, NATOMS: int, natoms: int,

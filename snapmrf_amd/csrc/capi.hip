@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace mrf {
@@ -291,6 +292,92 @@ extern "C" int mrf_match_ex(void *stream, int mode, const mrf_complex *d_atoms,
     for (void *p : {(void *)d_vs, scratch})
         if (p) (void)hipFreeAsync(p, s);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 matching without device allocation (mrf_match_ex_noalloc)
+
+// Scratch of mrf_match_ex_noalloc, one copy per GPU for the life of the
+// process: the B1 slice of each voxel, the per-voxel (score, index) keys, the
+// voxel permutation, the slice counts and the tile list of one FP32 matching
+// call.  2^20 voxels over 1,024 B1 slices need about 16.2 MiB of it.
+static constexpr size_t kNoAllocScratchBytes = (size_t)20 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace mrf {
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices];
+
+// Device address of the current device's copy of `scratch`, looked up on the
+// first call on each device (the array's host-side name is not a device
+// address and must never reach a kernel).
+int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return mrf_fail(MRF_E_GPU, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return mrf_fail(MRF_E_UNSUPPORTED, "device %d: the static scratch is tracked for devices "
+                        "0..%d", dev, kNoAllocMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return mrf_fail(MRF_E_GPU, "address of the static scratch: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+}  // namespace mrf
+
+extern "C" int mrf_match_ex_noalloc(void *stream, const mrf_complex *d_atoms,
+                                    const float *d_params, size_t T, size_t NATOMS,
+                                    size_t natoms, const mrf_complex *d_sig, size_t V,
+                                    const float *d_b1map, float *d_maps, int32_t *d_idx)
+{
+    mrf_clear_error();
+    if (!d_atoms || !d_params || (V && (!d_sig || !d_maps)))
+        return mrf_fail(MRF_E_ARG, "null pointer argument");
+    if (T == 0 || T > 0x7fffffff) return mrf_fail(MRF_E_ARG, "bad T");
+    if (natoms == 0 || NATOMS == 0 || NATOMS % natoms)
+        return mrf_fail(MRF_E_ARG, "NATOMS (%zu) must be a positive multiple of natoms (%zu)",
+                        NATOMS, natoms);
+    if (int rc = check_device()) return rc;
+    if (V == 0) return MRF_OK;
+    void *base = nullptr;
+    if (int rc = noalloc_scratch(&base)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int n_slices = (int)(NATOMS / natoms);
+    MatchLaunch l{};
+    l.atoms = reinterpret_cast<const float2 *>(d_atoms);
+    l.params = d_params;
+    l.sig = reinterpret_cast<const float2 *>(d_sig);
+    l.natoms = natoms;
+    l.n_slices = n_slices;
+    l.T = (int)T;
+    l.V = V;
+    l.mode = MRF_MATCH_FP32;
+    l.maps = d_maps;
+    l.idx = d_idx;
+    // [voxel slices | match scratch], both 256-byte aligned
+    const size_t vs_bytes = d_b1map ? (V * sizeof(int32_t) + 255) / 256 * 256 : 0;
+    int32_t *d_vs = d_b1map ? (int32_t *)base : nullptr;
+    l.vox_slice = d_vs;
+    const size_t need = vs_bytes + match_scratch_bytes(l);
+    if (need > kNoAllocScratchBytes)
+        return mrf_fail(MRF_E_ARG, "%zu voxels over %d slices need %zu bytes of scratch; the "
+                        "static scratch holds %zu (2^20 voxels over 1,024 slices fit)", V,
+                        d_b1map ? n_slices : 1, need, kNoAllocScratchBytes);
+    if (d_b1map)
+        if (int rc = launch_b1_select(d_b1map, d_params, natoms, n_slices, V, d_vs, s)) return rc;
+    return launch_match(l, (unsigned char *)base + vs_bytes, s);
 }
 
 extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const float *h_mrf,
