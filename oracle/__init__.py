@@ -30,7 +30,9 @@ def build() -> None:
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # ORACLE_LIB selects another build of the same sources (the
+        # sanitizer build liboracle_san.so)
+        path = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)

@@ -14,8 +14,10 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsnapmrf.so")
-CLI_PATH = os.path.join(_HERE, "bin", "mrf")
+# SNAPMRF_LIB / SNAPMRF_CLI select another build of the same sources (the
+# sanitizer build in snapmrf_amd/_san/)
+LIB_PATH = os.environ.get("SNAPMRF_LIB") or os.path.join(_HERE, "libsnapmrf.so")
+CLI_PATH = os.environ.get("SNAPMRF_CLI") or os.path.join(_HERE, "bin", "mrf")
 
 MRF_MATCH_FP32 = 0
 MRF_MATCH_BF16_TOP8 = 1

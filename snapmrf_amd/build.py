@@ -1,7 +1,9 @@
 """Builds the in-tree gfx950 library ``snapmrf_amd/libsnapmrf.so`` and the
 ``mrf`` command-line tool ``snapmrf_amd/bin/mrf``.
 
-    python -m snapmrf_amd.build          # or snapmrf_amd.build.build()
+    python -m snapmrf_amd.build            # or snapmrf_amd.build.build()
+    python -m snapmrf_amd.build --sanitize # also snapmrf_amd/_san/: host code
+                                           # under ASan + UBSan (see build_sanitized)
 
 Every HIP translation unit is compiled by hipcc for gfx950 only, with packed
 FP32 instructions disabled; host-only C++ is compiled with -ffp-contract=off
@@ -86,6 +88,40 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+SAN = os.path.join(HERE, "_san")
+SAN_LIB = os.path.join(SAN, "libsnapmrf.so")
+SAN_CLI = os.path.join(SAN, "mrf")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+             "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(verbose: bool = False) -> str:
+    """The library and CLI with their host-only C++ (host.cpp: CSV, grid
+    strings, RA I/O; mrf_main.cpp: the CLI) compiled by g++ under
+    AddressSanitizer + UndefinedBehaviorSanitizer, into snapmrf_amd/_san/.
+    The HIP translation units are linked in from the regular build without
+    instrumentation (no GPU code is sanitized).  Load the library with
+    SNAPMRF_LIB / SNAPMRF_CLI pointing here and libasan + libubsan preloaded."""
+    hip_objs = [os.path.join(OBJ, src + ".o") for src in HIP_SOURCES]
+    if not all(os.path.exists(o) for o in hip_objs):
+        build()
+    os.makedirs(SAN, exist_ok=True)
+    host_o = os.path.join(SAN, "host.o")
+    cli_o = os.path.join(SAN, "mrf_main.o")
+    _run(["g++", "-std=c++17", "-O1", "-fPIC", "-ffp-contract=off", "-Wall", *SAN_FLAGS,
+          f"-I{INCLUDE}", f"-I{CSRC}", "-c", "-o", host_o, os.path.join(CSRC, "host.cpp")])
+    _run(["g++", "-std=c++17", "-O1", "-Wall", *SAN_FLAGS, f"-I{INCLUDE}",
+          "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", cli_o,
+          os.path.join(CSRC, "mrf_main.cpp")])
+    _run(["g++", "-shared", *SAN_FLAGS, "-o", SAN_LIB, host_o, *hip_objs,
+          "-Wl,-soname,libsnapmrf.so", "-L/opt/rocm/lib", "-lamdhip64", *LIBS])
+    _run(["g++", *SAN_FLAGS, "-o", SAN_CLI, cli_o, f"-L{SAN}", "-lsnapmrf",
+          "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    if verbose:
+        print(f"built {SAN_LIB} and {SAN_CLI} (host code under ASan + UBSan)")
+    return SAN_LIB
+
+
 def _compile_cli() -> str:
     os.makedirs(OBJ, exist_ok=True)
     out = os.path.join(OBJ, "mrf_main.o")
@@ -96,3 +132,5 @@ def _compile_cli() -> str:
 
 if __name__ == "__main__":
     build(verbose=True)
+    if "--sanitize" in sys.argv[1:]:
+        build_sanitized(verbose=True)
