@@ -24,7 +24,7 @@
 // with H = E2(TE) E2(TR-TE) e^{i(theta_TE + theta_TR-TE)} e^{i(phi_n - phi_{n+1})}.
 // This is algebraically the reference's RF, decay(TE), phase(TE), sample,
 // phase(TR-TE), dephase, decay(TR-TE) sequence (src/functions.cu:975-1009),
-// at 26 VALU operations per state instead of the 102 flops written.
+// at 24 VALU operations per state instead of the 102 flops written.
 //
 // The SR/SE/TSE programs (and SSFP/FISP when `generic` is requested) run the
 // reference's step order unfused (src/functions.cu:687-933).
@@ -190,6 +190,14 @@ struct FusedCtx {
 
 // One TR of the fused frame-form step with the F- array offset R; returns
 // the sample E2(TE) e^{i theta_TE} e^{i phi_n} P+_0.
+//
+// The zero-phase RF is evaluated through S = A + B and D = A - B (A, B, C =
+// F~+, F~-, Z of a state): since (1+cos a)/2 + (1-cos a)/2 = 1 and their
+// difference is cos a,
+//     P+ = (S + W)/2,  P- = (S - W)/2,  W = cos(a) D + 2 sin(a) (C.y, -C.x),
+//     Q  = cos(a) C + sin(a)/2 (D.y, -D.x),
+// and the exact factor 1/2 is folded into H and into the sample factor.  That
+// is 24 operations per state (20 with a real H), 10 of them additions.
 template <int M, int R, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
 __device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
                                              const FusedCtx &x, const EpgArgs &a)
@@ -204,38 +212,40 @@ __device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
         sincos_rev(x.b0r * (double)c.te, &sa_, &ca_);
         sincos_rev(x.b0r * (double)c.tmte, &sb_, &cb_);
         const float2 e = cmul(make_float2(ca_, sa_), make_float2(cb_, sb_));
-        const float g = ea * eb;
+        const float g = 0.5f * ea * eb;
         const float2 er = cmul(e, make_float2(c.rr, c.ri));
         H = make_float2(g * er.x, g * er.y);
         const float2 ez = cmul(make_float2(ca_, sa_), make_float2(c.zr, c.zi));
-        csave = make_float2(ea * ez.x, ea * ez.y);
+        const float h = 0.5f * ea;
+        csave = make_float2(h * ez.x, h * ez.y);
     } else {
-        const float g = ea * eb;
+        const float g = 0.5f * ea * eb;
         H = HREAL ? make_float2(g, 0.f) : make_float2(g * c.rr, g * c.ri);
-        csave = make_float2(ea * c.zr, ea * c.zi);
+        const float h = 0.5f * ea;
+        csave = make_float2(h * c.zr, h * c.zi);
     }
-    const float ca2 = c.ca2, sa2 = c.sa2, sa = c.sa;
+    const float ca = c.ca, s2 = 2.0f * c.sa;
     const float eca = E1 * c.ca, eh = E1 * 0.5f * c.sa;
 
-    float2 top = make_float2(0.f, 0.f), d0 = top, fp0 = top, p0 = top;
+    float2 top = make_float2(0.f, 0.f), d0 = top, fp0 = top, u0 = top;
 #pragma unroll
     for (int i = M - 1; i >= 0; i--) {
         const float2 A = s.fp[i], B = s.fm[i + R], C = s.z[i];
-        const float2 pp = make_float2(ca2 * A.x + sa2 * B.x + sa * C.y,
-                                      ca2 * A.y + sa2 * B.y - sa * C.x);
-        const float2 pm = make_float2(sa2 * A.x + ca2 * B.x - sa * C.y,
-                                      sa2 * A.y + ca2 * B.y + sa * C.x);
+        const float sx = A.x + B.x, sy = A.y + B.y;
         const float dx = A.x - B.x, dy = A.y - B.y;
+        const float wx = ca * dx + s2 * C.y, wy = ca * dy - s2 * C.x;
+        const float2 U = make_float2(sx + wx, sy + wy);    // 2 P+
+        const float2 V = make_float2(sx - wx, sy - wy);    // 2 P-
         s.z[i] = make_float2(eca * C.x + eh * dy, eca * C.y - eh * dx);
-        const float2 hp = HREAL ? make_float2(H.x * pp.x, H.x * pp.y) : cmul(H, pp);
-        const float2 hm = HREAL ? make_float2(H.x * pm.x, H.x * pm.y) : cmulc(H, pm);
+        const float2 hp = HREAL ? make_float2(H.x * U.x, H.x * U.y) : cmul(H, U);
+        const float2 hm = HREAL ? make_float2(H.x * V.x, H.x * V.y) : cmulc(H, V);
         if (i == M - 1) top = hp;
         else s.fp[i + 1] = hp;            // new F+_{i+1}
         if (i >= 1) s.fm[i + R] = hm;     // new F-_{i-1}, at index (i-1) + (R+1)
         else d0 = hm;
         if (i == 1)
-            fp0 = HREAL ? make_float2(H.x * pm.x, -H.x * pm.y) : cmul(H, make_float2(pm.x, -pm.y));
-        if (i == 0) p0 = pp;
+            fp0 = HREAL ? make_float2(H.x * V.x, -H.x * V.y) : cmul(H, make_float2(V.x, -V.y));
+        if (i == 0) u0 = U;
     }
     if (x.j == 0) s.z[0].x += 1.0f - E1;
     if (MULTI) {
@@ -252,7 +262,7 @@ __device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
         for (int i = 0; i < M; i++)
             if (i == a.istar && x.j == a.jstar) s.fm[i + R + 1] = make_float2(0.f, 0.f);
     }
-    return cmul(csave, p0);
+    return cmul(csave, u0);
 }
 
 template <int M, int SH>
