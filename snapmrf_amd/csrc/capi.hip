@@ -602,14 +602,16 @@ void build_dtr(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<DTr> 
     for (size_t i = 0; i < nreps; i++) {
         const double fa = (double)h_mrf[i] * (60.0 * M_PI / 180.0);
         const double ph = (double)h_mrf[NREPS + i] * (M_PI / 180.0);
+        const double ph_next =
+            i + 1 < nreps ? (double)h_mrf[NREPS + i + 1] * (M_PI / 180.0) : ph;
         const double tr = (double)h_mrf[2 * NREPS + i] + 16.0;
         const double te = (double)h_mrf[3 * NREPS + i] + 3.5;
         DTr d;
         d.al = (float)fa;
         d.cph = (float)cos(ph);
         d.sph = (float)sin(ph);
-        d.c2p = (float)cos(2.0 * ph);
-        d.s2p = (float)sin(2.0 * ph);
+        d.rr = (float)cos(ph - ph_next);
+        d.ri = (float)sin(ph - ph_next);
         d.te = (float)te;
         d.t2 = (float)(tr - te);
         d.pad = 0.f;
@@ -637,8 +639,8 @@ void build_gn_coef(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<T
         c.te = d[i].te;
         c.tmte = d[i].t2;
         c.tr = d[i].te + d[i].t2;
-        c.s2p = d[i].s2p;
-        c.c2p = d[i].c2p;
+        c.s2p = (float)sin(2.0 * ph);
+        c.c2p = (float)cos(2.0 * ph);
         c.pad[0] = d[i].al;
         out[i] = c;
     }

@@ -16,6 +16,14 @@
 // the dephasing shift is a cross-lane move inside the group.  Per TR n:
 //   RF(alpha_n B1, phi_n) -> relax(TE_n) -> phase(TE_n) -> s_n = F+_0 (lane 0)
 //   -> phase(TR_n - TE_n) -> dephase -> relax(TR_n - TE_n)
+// TR n runs in the frame of its RF phase, x~ = P_n^-1 x with
+// P_n = diag(e^{i phi_n}, e^{-i phi_n}, 1): relaxation, precession and the
+// gradient shift commute with P_n, so the RF there is the zero-phase matrix
+// and the change to frame n+1, F+ e^{i(phi_n - phi_{n+1})} and F- times its
+// conjugate, has the form of the TR - TE precession and is folded into its
+// factor.  Readouts go back to true coordinates by e^{i phi_n}, the
+// adjoint's cotangents by e^{-i phi_n}, and the initial state (the inversion
+// at phase 0) enters frame 0 by e^{-/+ i phi_0}.
 // Arithmetic is FP32; FP64 is used only for the 4x4 Gauss-Newton solve and
 // the Fisher-matrix inverse.
 #include "kernels.h"
@@ -78,40 +86,61 @@ __device__ __forceinline__ float group_sum(float v)
     return v;
 }
 
-// RF matrix entries for angle a (or its a-derivatives)
+// The zero-phase RF matrix M0(a) = [[ca2, sa2, -i sa], [sa2, ca2, i sa],
+// [-i sa/2, i sa/2, ca]] (apply_rf_pulse, functions.cu:483-528, at phi = 0;
+// ca2 = (1 + cos a)/2, sa2 = (1 - cos a)/2) or one of its a-derivatives,
+// stored as hs = (ca2 + sa2)/2, hd = (ca2 - sa2)/2, s = sa, c = ca: with
+// S = F+ + F-, D = F+ - F-,
+//     F+' = hs S + W,  F-' = hs S - W,  W = hd D + s (Z.y, -Z.x),
+//     Z'  = c Z + (s/2) (D.y, -D.x).
+// hs is 1/2 for M0 and 0 for its derivatives (HS = false drops S).
 struct C4 {
-    float ca2, sa2, sa, ca;
+    float hs, hd, s, c;
 };
 
-// y' = M y with M(a, phi) of apply_rf_pulse (functions.cu:483-528)
-__device__ __forceinline__ void rf3(const C4 &c, float2 zz, float2 z2, float2 fp, float2 fm,
-                                    float2 z, float2 &op, float2 &om, float2 &oz)
+template <bool HS>
+__device__ __forceinline__ void rf3(const C4 &m, float2 fp, float2 fm, float2 z, float2 &op,
+                                    float2 &om, float2 &oz)
 {
-    const float2 zc = c_conj(zz);
-    op = c_add(c_add(c_scale(fp, c.ca2), c_scale(c_mul(z2, fm), c.sa2)),
-               c_scale(c_i(c_mul(zz, z)), -c.sa));
-    om = c_add(c_add(c_scale(c_mul(c_conj(z2), fp), c.sa2), c_scale(fm, c.ca2)),
-               c_scale(c_i(c_mul(zc, z)), c.sa));
-    oz = c_add(c_add(c_scale(c_i(c_mul(zc, fp)), -0.5f * c.sa), c_scale(c_i(c_mul(zz, fm)), 0.5f * c.sa)),
-               c_scale(z, c.ca));
+    const float dx = fp.x - fm.x, dy = fp.y - fm.y;
+    const float wx = m.hd * dx + m.s * z.y, wy = m.hd * dy - m.s * z.x;
+    if (HS) {
+        const float sx = fp.x + fm.x, sy = fp.y + fm.y;
+        op = {m.hs * sx + wx, m.hs * sy + wy};
+        om = {m.hs * sx - wx, m.hs * sy - wy};
+    } else {
+        op = {wx, wy};
+        om = {-wx, -wy};
+    }
+    const float h = 0.5f * m.s;
+    oz = {m.c * z.x + h * dy, m.c * z.y - h * dx};
 }
 
-// l = M^H l'
-__device__ __forceinline__ void rf3_adj(const C4 &c, float2 zz, float2 z2, float2 lp, float2 lm,
-                                        float2 lz, float2 &op, float2 &om, float2 &oz)
+// l = M^H l': F+ = hs S + W, F- = hs S - W with S = l+ + l-, D = l+ - l-,
+// W = hd D + (s/2) (-lz.y, lz.x); Z = c lz + s (-D.y, D.x)
+template <bool HS>
+__device__ __forceinline__ void rf3_adj(const C4 &m, float2 lp, float2 lm, float2 lz, float2 &op,
+                                        float2 &om, float2 &oz)
 {
-    const float2 zc = c_conj(zz);
-    op = c_add(c_add(c_scale(lp, c.ca2), c_scale(c_mul(z2, lm), c.sa2)),
-               c_scale(c_i(c_mul(zz, lz)), 0.5f * c.sa));
-    om = c_add(c_add(c_scale(c_mul(c_conj(z2), lp), c.sa2), c_scale(lm, c.ca2)),
-               c_scale(c_i(c_mul(zc, lz)), -0.5f * c.sa));
-    oz = c_add(c_add(c_scale(c_i(c_mul(zc, lp)), c.sa), c_scale(c_i(c_mul(zz, lm)), -c.sa)),
-               c_scale(lz, c.ca));
+    const float dx = lp.x - lm.x, dy = lp.y - lm.y;
+    const float h = 0.5f * m.s;
+    const float wx = m.hd * dx - h * lz.y, wy = m.hd * dy + h * lz.x;
+    if (HS) {
+        const float sx = lp.x + lm.x, sy = lp.y + lm.y;
+        op = {m.hs * sx + wx, m.hs * sy + wy};
+        om = {m.hs * sx - wx, m.hs * sy - wy};
+    } else {
+        op = {wx, wy};
+        om = {-wx, -wy};
+    }
+    oz = {m.c * lz.x - m.s * dy, m.c * lz.y + m.s * dx};
 }
 
-__device__ __forceinline__ C4 rf_c(float sa, float ca) { return {0.5f * (1.f + ca), 0.5f * (1.f - ca), sa, ca}; }
-__device__ __forceinline__ C4 rf_d1(float sa, float ca) { return {-0.5f * sa, 0.5f * sa, ca, -sa}; }
-__device__ __forceinline__ C4 rf_d2(float sa, float ca) { return {-0.5f * ca, 0.5f * ca, -sa, -ca}; }
+// M0(a), dM0/da, d2M0/da2 (the entries' derivatives: ca2' = -sa/2, sa2' = sa/2,
+// sa' = ca, ca' = -sa)
+__device__ __forceinline__ C4 rf_c(float sa, float ca) { return {0.5f, 0.5f * ca, sa, ca}; }
+__device__ __forceinline__ C4 rf_d1(float sa, float ca) { return {0.f, -0.5f * sa, ca, -sa}; }
+__device__ __forceinline__ C4 rf_d2(float sa, float ca) { return {0.f, -0.5f * ca, -sa, -ca}; }
 
 struct Atom {
     float T1, T2, B0, B1;
@@ -128,10 +157,10 @@ struct Aug {
 // Per-TR coefficients, identical on all lanes of the wave.
 struct TrK {
     float sa, ca;            // sin/cos of alpha_n B1
-    float2 zz, z2;
+    float2 ro;               // e^{i phi_n}: frame n -> true coordinates
     float al;
     float e1a, e2a, e1b, e2b;     // relaxation over TE and TR - TE
-    float2 pa, pb;                // e^{i beta TE}, e^{i beta (TR - TE)}
+    float2 pa, pb;                // e^{i beta TE}, e^{i beta (TR - TE)} e^{i(phi_n - phi_{n+1})}
     float te, t2;
 };
 
@@ -140,8 +169,7 @@ __device__ __forceinline__ TrK tr_coef(const DTr &q, const Atom &p)
     TrK c;
     c.al = q.al;
     sincosf(q.al * p.B1, &c.sa, &c.ca);
-    c.zz = {q.cph, q.sph};
-    c.z2 = {q.c2p, q.s2p};
+    c.ro = {q.cph, q.sph};
     c.te = q.te;
     c.t2 = q.t2;
     c.e1a = expf(-q.te * p.iT1);
@@ -153,7 +181,7 @@ __device__ __forceinline__ TrK tr_coef(const DTr &q, const Atom &p)
     sincosf(beta * q.te, &s, &co);
     c.pa = {co, s};
     sincosf(beta * q.t2, &s, &co);
-    c.pb = {co, s};
+    c.pb = c_mul({co, s}, {q.rr, q.ri});   // with the change to frame n+1
     return c;
 }
 
@@ -221,8 +249,7 @@ __device__ __forceinline__ TrK coef(CoefCache<SUB> &cc, const DTr *seq, int T, c
     c.al = q.al;
     c.sa = e.sa;
     c.ca = e.ca;
-    c.zz = {q.cph, q.sph};
-    c.z2 = {q.c2p, q.s2p};
+    c.ro = {q.cph, q.sph};
     c.te = q.te;
     c.t2 = q.t2;
     c.e1a = e.e1a;
@@ -256,12 +283,12 @@ __device__ __forceinline__ void f_rf(Aug<ND> &y, const TrK &c)
     const C4 m = rf_c(c.sa, c.ca);
     float2 sp, sm, sz;
     if constexpr (ND > 0) {
-        rf3(rf_d1(c.sa, c.ca), c.zz, c.z2, y.fp[0], y.fm[0], y.z[0], sp, sm, sz);
+        rf3<false>(rf_d1(c.sa, c.ca), y.fp[0], y.fm[0], y.z[0], sp, sm, sz);
     }
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
         float2 a, b, d;
-        rf3(m, c.zz, c.z2, y.fp[j], y.fm[j], y.z[j], a, b, d);
+        rf3<true>(m, y.fp[j], y.fm[j], y.z[j], a, b, d);
         y.fp[j] = a;
         y.fm[j] = b;
         y.z[j] = d;
@@ -346,9 +373,10 @@ __device__ __forceinline__ void f_tr(Aug<ND> &y, const TrK &c, const Atom &p, in
 }
 
 // init_rf_pulse(180 deg, 0 deg, TI) (functions.cu:456-481) on lane 0, with
-// the tangents of its T1/T2 dependence.
+// the tangents of its T1/T2 dependence, in the frame of TR 0 (ro0 =
+// e^{i phi_0}: F+ times conj(ro0), F- times ro0).
 template <int ND>
-__device__ __forceinline__ void f_init(Aug<ND> &y, const Atom &p, int k)
+__device__ __forceinline__ void f_init(Aug<ND> &y, const Atom &p, int k, float2 ro0)
 {
 #pragma unroll
     for (int j = 0; j <= ND; j++) y.fp[j] = y.fm[j] = y.z[j] = {0.f, 0.f};
@@ -357,17 +385,19 @@ __device__ __forceinline__ void f_init(Aug<ND> &y, const Atom &p, int k)
     const float sa0 = sinf(a0), ca0 = cosf(a0);
     const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
     const float2 tmp = {0.f, e2 * sa0};
-    y.fp[0] = c_conj(tmp);
-    y.fm[0] = tmp;
+    y.fp[0] = c_mul(c_conj(ro0), c_conj(tmp));
+    y.fm[0] = c_mul(ro0, tmp);
     y.z[0] = {e1 * ca0 + 1.f - e1, 0.f};
     if constexpr (ND > 0) {
         const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
         y.z[1] = {e1d * (ca0 - 1.f), 0.f};
         const float2 t2 = {0.f, e2d * sa0};
-        y.fp[2] = c_conj(t2);
-        y.fm[2] = t2;
+        y.fp[2] = c_mul(c_conj(ro0), c_conj(t2));
+        y.fm[2] = c_mul(ro0, t2);
     }
 }
+
+__device__ __forceinline__ float2 frame0(const DTr *seq) { return {seq[0].cph, seq[0].sph}; }
 
 // --- reverse operators ----------------------------------------------------
 // l holds the adjoint of the operator's output on entry and of its input on
@@ -485,7 +515,7 @@ __device__ __forceinline__ void r_rf(Aug<ND> &l, const Aug<ND> &x, const TrK &c,
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
         float2 a, b, d;
-        rf3(m1, c.zz, c.z2, x.fp[j], x.fm[j], x.z[j], a, b, d);
+        rf3<false>(m1, x.fp[j], x.fm[j], x.z[j], a, b, d);
         g_a += re_dot(l.fp[j], a) + re_dot(l.fm[j], b) + re_dot(l.z[j], d);
         if (ND > 0 && j == 0) {
             // explicit alpha of the B1 source alpha M_a x0
@@ -495,16 +525,16 @@ __device__ __forceinline__ void r_rf(Aug<ND> &l, const Aug<ND> &x, const TrK &c,
     float2 sp, sm, sz;
     if constexpr (ND > 0) {
         float2 a, b, d;
-        rf3(rf_d2(c.sa, c.ca), c.zz, c.z2, x.fp[0], x.fm[0], x.z[0], a, b, d);
+        rf3<false>(rf_d2(c.sa, c.ca), x.fp[0], x.fm[0], x.z[0], a, b, d);
         g_a += c.al * (re_dot(l.fp[4], a) + re_dot(l.fm[4], b) + re_dot(l.z[4], d));
-        rf3_adj(m1, c.zz, c.z2, l.fp[4], l.fm[4], l.z[4], sp, sm, sz);
+        rf3_adj<false>(m1, l.fp[4], l.fm[4], l.z[4], sp, sm, sz);
     }
     *ga += B1 * g_a + g_al;
     if constexpr (ND == 0) gth[3] += c.al * g_a;
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
         float2 a, b, d;
-        rf3_adj(m, c.zz, c.z2, l.fp[j], l.fm[j], l.z[j], a, b, d);
+        rf3_adj<true>(m, l.fp[j], l.fm[j], l.z[j], a, b, d);
         l.fp[j] = a;
         l.fm[j] = b;
         l.z[j] = d;
@@ -584,15 +614,15 @@ __global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T,
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
     float2 *rd = read_stage<SUB>(read_lds);
     Aug<4> y;
-    f_init(y, p, k);
+    f_init(y, p, k, frame0(seq));
     for (int t = 0; t < T; t++) {
         const TrK c = coef<SUB>(cc, seq, T, p, t, k);
         Aug<4> r;
         f_tr<4, SUB>(y, c, p, k, w, &r);
         const int j = t % SUB;
-        if (k == 0)
+        if (k == 0)   // readouts in true coordinates
 #pragma unroll
-            for (int d = 0; d < 5; d++) rd[j * 5 + d] = r.fp[d];
+            for (int d = 0; d < 5; d++) rd[j * 5 + d] = c_mul(c.ro, r.fp[d]);
         if (j == SUB - 1 || t == T - 1) {
             // lane k stores TR t - j + k: coalesced rows of s and J
             wave_sync();
@@ -679,9 +709,11 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     const Atom p = load_atom(A.theta, a);
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
 
-    // forward sweep: checkpoints every kCkpt TRs; Fisher sums (ND = 4)
+    // forward sweep: checkpoints every kCkpt TRs; Fisher sums (ND = 4, from
+    // frame readouts: Re(J^H J) does not see the frame phases)
+    const float2 ro0 = frame0(A.seq);
     Aug<ND> y;
-    f_init(y, p, k);
+    f_init(y, p, k, ro0);
     float F[10];
 #pragma unroll
     for (int i = 0; i < 10; i++) F[i] = 0.f;
@@ -793,7 +825,8 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
                             l.fp[1 + e] = c_add(l.fp[1 + e], cg);
                         }
                     } else {
-                        l.fp[0] = c_add(l.fp[0], A.cot[a * T + t]);
+                        // L = Re conj(v) s with s = ro s~: cotangent conj(ro) v
+                        l.fp[0] = c_add(l.fp[0], c_mul(c_conj(c.ro), A.cot[a * T + t]));
                     }
                 }
             }
@@ -826,7 +859,8 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
             const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
             gth[0] += l.z[0].x * e1d * (ca0 - 1.f);
             const float2 t2 = {0.f, e2d * sa0};
-            gth[1] += re_dot(l.fp[0], c_conj(t2)) + re_dot(l.fm[0], t2);
+            gth[1] += re_dot(l.fp[0], c_mul(c_conj(ro0), c_conj(t2))) +
+                      re_dot(l.fm[0], c_mul(ro0, t2));
         }
 #pragma unroll
         for (int d = 0; d < 4; d++) gth[d] = group_sum<SUB>(gth[d]);
@@ -894,14 +928,15 @@ __device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const flo
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);   // p differs per pass
     float2 *rd = read_stage<SUB>(read_lds);
     Aug<ND> y;
-    f_init(y, p, k);
+    f_init(y, p, k, frame0(seq));
     for (int t = 0; t < T; t++) {
         Aug<ND> r;
-        f_tr<ND, SUB>(y, coef<SUB>(cc, seq, T, p, t, k), p, k, w, &r);
+        const TrK c = coef<SUB>(cc, seq, T, p, t, k);
+        f_tr<ND, SUB>(y, c, p, k, w, &r);
         const int j = t % SUB;
-        if (k == 0)
+        if (k == 0)   // readouts in true coordinates
 #pragma unroll
-            for (int d = 0; d <= ND; d++) rd[j * (1 + ND) + d] = r.fp[d];
+            for (int d = 0; d <= ND; d++) rd[j * (1 + ND) + d] = c_mul(c.ro, r.fp[d]);
         if (j == SUB - 1 || t == T - 1) {
             // lane k folds TR t - j + k into its partial sums
             wave_sync();

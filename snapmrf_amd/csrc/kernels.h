@@ -106,11 +106,13 @@ int launch_lowp_prepare(const float2 *src, const int32_t *perm, size_t rows, int
 
 // Differentiation form (grad.hip) --------------------------------------
 
-// per-TR table of the continuous EPG-SSFP model (unit transform in double)
+// per-TR table of the continuous EPG-SSFP model (unit transform in double).
+// The kernels run TR n in the frame of its RF phase (F+ e^{-i phi_n},
+// F- e^{i phi_n}), where the RF has zero phase.
 struct DTr {
     float al;        // flip angle (rad) before B1
-    float cph, sph;  // e^{i phi}
-    float c2p, s2p;  // e^{2 i phi}
+    float cph, sph;  // e^{i phi_n}: readouts of TR n back to true coordinates
+    float rr, ri;    // e^{i (phi_n - phi_{n+1})} (phi_T := phi_{T-1}): frame n -> n+1
     float te, t2;    // TE and TR - TE (ms)
     float pad;
 };

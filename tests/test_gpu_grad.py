@@ -404,3 +404,84 @@ def test_gauss_newton_zero_iterations(gpu):
     xx = np.sum(np.abs(x64) ** 2, axis=1)
     assert np.allclose(d_pd.cpu().numpy(), sx / np.sqrt(ss), rtol=1e-5)
     assert np.all(np.abs(d_cost.cpu().numpy()[:, 0] - (xx - sx ** 2 / ss)) <= 1e-5 * xx)
+
+
+def _raw_phase(T, kind, seed=17):
+    """Sequences with RF phase: 'ramp' (0 -> 137 degrees plus noise, every TR
+    different) or 'const' (30 degrees throughout)."""
+    sq = synth.sequence(T, seed=seed)
+    rng = np.random.default_rng(seed)
+    if kind == "ramp":
+        sq["phi_deg"] = np.linspace(0.0, 137.0, T) + rng.uniform(-20.0, 20.0, T)
+    else:
+        sq["phi_deg"] = np.full(T, 30.0)
+    return synth.csv_columns(sq, "epg_ssfp").astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["ramp", "const"])
+def test_rf_phase_jacobian_and_adjoint_match_oracle(gpu, kind):
+    """The kernels run each TR in the frame of its RF phase (zero-phase RF,
+    frame changes folded into the TR - TE precession, readouts and
+    cotangents rotated by e^{+/- i phi_n}, the initial state by e^{-/+ i phi_0}):
+    signals, Jacobians, theta and sequence adjoints against the FP64 oracle,
+    which applies the RF with its phase as written (src/functions.cu:483-528)."""
+    T, w = 300, 24
+    raw = _raw_phase(T, kind)
+    s, J = _jac(gpu, raw, THETAS, w)
+    n = len(THETAS)
+    rng = np.random.default_rng(21)
+    v = (rng.normal(size=(n, T)) + 1j * rng.normal(size=(n, T))).astype(np.complex64)
+    g_th = gpu.empty((n, 4), device="cuda")
+    g_seq = gpu.zeros((2, T), device="cuda")
+    sm.epg_adjoint(raw, gpu.from_numpy(THETAS).cuda(), n, w, gpu.from_numpy(v).cuda(),
+                   d_grad_theta=g_th, d_grad_seq=g_seq)
+    gpu.cuda.synchronize()
+    g_th, g_seq = g_th.cpu().numpy(), g_seq.cpu().numpy()
+    sfa, str_ = np.zeros(T), np.zeros(T)
+    for a, th in enumerate(THETAS.astype(np.float64)):
+        s64, J64 = oracle.dm_jacobian(raw, th, w)
+        assert _rel(s[a], s64) < TOL, a
+        for d in range(4):
+            assert _rel(J[a, d], J64[d]) < TOL, (a, d)
+        g, gfa, gtr = oracle.dm_adjoint(raw, th, w, v[a].astype(np.complex128))
+        assert np.all(np.abs(g_th[a] - g) <= TOL * np.abs(g) + 1e-6 * np.abs(g).max()), (a, g_th[a], g)
+        sfa += gfa
+        str_ += gtr
+    assert np.abs(g_seq[0] - sfa).max() <= TOL * np.abs(sfa).max()
+    assert np.abs(g_seq[1] - str_).max() <= TOL * np.abs(str_).max()
+
+
+def test_rf_phase_crb_and_gauss_newton(gpu):
+    """CRB loss with a varying RF phase against the FP64 oracle, and
+    Gauss-Newton on noise-free signals of that sequence recovering off-grid
+    truths (its sums use readouts in true coordinates)."""
+    T, w = 300, 24
+    raw = _raw_phase(T, "ramp", seed=23)
+    rng = np.random.default_rng(25)
+    n = 16
+    tis = np.stack([rng.uniform(300, 2500, n), rng.uniform(30, 300, n), rng.uniform(-30, 30, n),
+                    rng.uniform(0.85, 1.15, n)], 1).astype(np.float32)
+    wts = np.array([1.0, 1.0, 0.01, 0.1], np.float32)
+    d_loss = gpu.empty(n, dtype=gpu.float64, device="cuda")
+    sm.crb_loss_grad(raw, gpu.from_numpy(tis).cuda(), n, w, wts, d_loss)
+    gpu.cuda.synchronize()
+    loss = d_loss.cpu().numpy()
+    for i in range(n):
+        want = oracle.dm_crb(raw, tis[i:i + 1].astype(np.float64), w, wts.astype(np.float64))
+        assert abs(loss[i] - want) <= 1e-3 * want, i
+    V, iters = 64, 25
+    truth = tis[rng.integers(0, n, V)]
+    s, _ = _jac(gpu, raw, truth, w)
+    x = (s * 0.7).astype(np.complex64)
+    start = truth * np.array([1.05, 0.95, 1.0, 1.0], np.float32) + np.array([0, 0, 1.0, 0.03], np.float32)
+    d_th = gpu.from_numpy(start.copy()).cuda()
+    d_cost = gpu.empty((V, iters + 1), device="cuda")
+    sm.gn_refine(raw, gpu.from_numpy(x).cuda(), V, w, iters, d_th, d_cost=d_cost)
+    gpu.cuda.synchronize()
+    th, cost = d_th.cpu().numpy(), d_cost.cpu().numpy()
+    assert np.all(np.diff(cost, axis=1) <= 0.0)
+    xx = np.sum(np.abs(x) ** 2, axis=1)
+    assert np.all(cost[:, -1] <= 1e-5 * xx)
+    err = np.abs(th - truth)
+    assert np.all(err[:, 0] <= 1e-3 * truth[:, 0]) and np.all(err[:, 1] <= 1e-3 * truth[:, 1])
+    assert np.all(err[:, 3] <= 1e-3)
