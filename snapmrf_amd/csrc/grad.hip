@@ -56,33 +56,66 @@ __device__ __forceinline__ float2 c_i(float2 a) { return {-a.y, a.x}; }      // 
 __device__ __forceinline__ float re_dot(float2 l, float2 v) { return l.x * v.x + l.y * v.y; }  // Re(conj(l) v)
 
 // Lane groups of SUB lanes (32 or 64) each hold one atom; shifts,
-// broadcasts and sums stay inside the group.
+// broadcasts and sums stay inside the group.  The shifts are whole-wave DPP
+// moves (wave_shr:1, wave_shl:1) on the VALU: the lanes that receive a value
+// from across a group boundary (k = 0 for the up shift, k = SUB - 1 for the
+// down shift) are the ones every caller overwrites (F+_0 <- conj(F-_0),
+// states k >= w - 1 zeroed, w <= SUB).
+__device__ __forceinline__ float wave_shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float wave_shl1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x130, 0xF, 0xF, true));
+}
 template <int SUB>
 __device__ __forceinline__ float2 shfl_up1(float2 v)
 {
-    return {__shfl_up(v.x, 1, SUB), __shfl_up(v.y, 1, SUB)};
+    return {wave_shr1(v.x), wave_shr1(v.y)};
 }
 template <int SUB>
 __device__ __forceinline__ float2 shfl_down1(float2 v)
 {
-    return {__shfl_down(v.x, 1, SUB), __shfl_down(v.y, 1, SUB)};
+    return {wave_shl1(v.x), wave_shl1(v.y)};
+}
+// lane 0 of the group (lanes 0 and 32 of the wave for SUB = 32)
+template <int SUB>
+__device__ __forceinline__ float bcast0(float v)
+{
+    const float a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 0));
+    if (SUB == 64) return a;
+    const float b = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32));
+    return (threadIdx.x & 32) ? b : a;
 }
 template <int SUB>
 __device__ __forceinline__ float2 bcast0(float2 v)
 {
-    return {__shfl(v.x, 0, SUB), __shfl(v.y, 0, SUB)};
-}
-template <int SUB>
-__device__ __forceinline__ float bcast0(float v)
-{
-    return __shfl(v, 0, SUB);
+    return {bcast0<SUB>(v.x), bcast0<SUB>(v.y)};
 }
 
+// v + the value of the lane given by a DPP pattern
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                                             0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+// Sum over the SUB lanes of the group, on every lane: pairs within quads
+// (quad_perm [1,0,3,2] and [2,3,0,1]), then 8-lane half rows (row_half_mirror)
+// and 16-lane rows (row_mirror) by DPP on the VALU, then across rows.
 template <int SUB>
 __device__ __forceinline__ float group_sum(float v)
 {
-#pragma unroll
-    for (int o = SUB / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, SUB);
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    v = dpp_add<0x141>(v);
+    v = dpp_add<0x140>(v);
+    if (SUB >= 32) v += __shfl_xor(v, 16, SUB);
+    if (SUB >= 64) v += __shfl_xor(v, 32, SUB);
     return v;
 }
 
@@ -783,9 +816,13 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
             st_store(seg + (size_t)(t - t0) * C, w, k, y);
             f_tr<ND, SUB>(y, coef<SUB>(cc, A.seq, T, p, t, k), p, k, w, nullptr);
         }
+        // the TR-start state of the next (earlier) TR is loaded while this
+        // TR is reversed
+        Aug<ND> ynext;
+        st_load(seg + (size_t)(t1 - 1 - t0) * C, w, k, ynext);
         for (int t = t1 - 1; t >= t0; t--) {
-            Aug<ND> yn;
-            st_load(seg + (size_t)(t - t0) * C, w, k, yn);
+            const Aug<ND> yn = ynext;
+            if (t > t0) st_load(seg + (size_t)(t - 1 - t0) * C, w, k, ynext);
             const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
             float ga = 0.f, gt = 0.f;
             // the TR's forward chain once: x1 = rf(yn), x2 = relax_a(x1),
