@@ -95,6 +95,21 @@ __device__ __forceinline__ float2 bcast0(float2 v)
 {
     return {bcast0<SUB>(v.x), bcast0<SUB>(v.y)};
 }
+// Entry i (uniform) of a per-group table held across the group's lanes: lane
+// k holds entry k in c0 and entry 32 + k in c1 (SUB = 32; SUB = 64 uses c0)
+template <int SUB>
+__device__ __forceinline__ float2 group_pick(float2 c0, float2 c1, int i)
+{
+    const float2 src = (SUB == 32 && i >= 32) ? c1 : c0;
+    const int li = SUB == 32 ? (i & 31) : i;
+    auto rl = [](float v, int lane) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+    };
+    const float2 a = {rl(src.x, li), rl(src.y, li)};
+    if (SUB == 64) return a;
+    const float2 b = {rl(src.x, li + 32), rl(src.y, li + 32)};
+    return (threadIdx.x & 32) ? b : a;
+}
 
 // v + the value of the lane given by a DPP pattern
 template <int CTRL>
@@ -808,9 +823,18 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     for (int j = 0; j <= ND; j++) l.fp[j] = l.fm[j] = l.z[j] = {0.f, 0.f};
     float gth[4] = {0.f, 0.f, 0.f, 0.f};
     const float beta = kCB0 * p.B0;
+    static_assert(kCkpt <= 64, "a segment's cotangents live in two registers per lane");
     for (int sgi = A.nck - 1; sgi >= 0; sgi--) {
         const int t0 = sgi * kCkpt;
         const int t1 = min(T, t0 + kCkpt);
+        // the segment's cotangents, loaded while its states are recomputed
+        // and read back per TR by readlane
+        float2 cv0 = {0.f, 0.f}, cv1 = {0.f, 0.f};
+        if constexpr (ND == 0) {
+            const float2 *cr = A.cot + a * T;
+            if (t0 + k < t1) cv0 = cr[t0 + k];
+            if (SUB == 32 && t0 + 32 + k < t1) cv1 = cr[t0 + 32 + k];
+        }
         st_load(ck + (size_t)sgi * C, w, k, y);
         for (int t = t0; t < t1; t++) {
             st_store(seg + (size_t)(t - t0) * C, w, k, y);
@@ -845,6 +869,8 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
             {
                 const Aug<ND> &x = x3;
                 r_phase(l, x, c.pb, c.t2, beta, &gt, gth);
+                float2 cot = {0.f, 0.f};
+                if constexpr (ND == 0) cot = group_pick<SUB>(cv0, cv1, t - t0);
                 if (k == 0) {
                     if constexpr (ND > 0) {
                         // dL/dJ_e = 2 sum_d G_de J_d
@@ -863,7 +889,7 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
                         }
                     } else {
                         // L = Re conj(v) s with s = ro s~: cotangent conj(ro) v
-                        l.fp[0] = c_add(l.fp[0], c_mul(c_conj(c.ro), A.cot[a * T + t]));
+                        l.fp[0] = c_add(l.fp[0], c_mul(c_conj(c.ro), cot));
                     }
                 }
             }
