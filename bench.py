@@ -124,6 +124,10 @@ def profile_run(sm, torch, np, synth, raw, T, w, d_params, A, natoms, params, d_
         fn()
         torch.cuda.synchronize()
         return float("nan")   # rates are not reported in profile mode
+    differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, natoms, stream,
+                    timed1, 0)
+    log("profile: differentiation done")
+    torch.cuda.empty_cache()
     precision_form(sm, torch, np, synth, raw, T, w, stream, 1, timed1, 0)
     log("profile: precision form done")
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
@@ -235,7 +239,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     kernels = {"bf16_top8": "match_lowp_kernel<1,32,3,2> + rescore_kernel<8,24>",
                "bf16_top16": "match_lowp_kernel<1,32,3,2> + rescore_kernel<16,24>",
                "bf16_top32": "match_lowp_kernel<1,32,6,1> + rescore_kernel<32,48>",
-               "bf16_top64": "match_lowp_kernel<1,16,8,1> + rescore_kernel<64,64>"}
+               "bf16_top64": "match_lowp_kernel<1,32,3,2> + rescore_kernel<64,24>"}
     ok = [n for n in kernels if res[n]["argmax_agree_fp32"] >= 0.999]
     best = max(ok, key=lambda n: res[n]["voxels_per_s"]) if ok else "bf16_top64"
     res["headline_mode"] = best

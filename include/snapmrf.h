@@ -210,10 +210,11 @@ int mrf_match_ex_noalloc(void *stream, const mrf_complex *d_atoms,
  * matched at once, and the running per-voxel best merges across blocks.
  * mode MRF_MATCH_FP32: d_maps / d_idx equal mrf_dict_ex followed by
  * mrf_match_ex (MRF_MATCH_FP32) on the same inputs, without the NATOMS x T
- * dictionary in memory.  Low-precision modes: each block is converted to
- * BF16 and its candidates are rescored against its complex64 rows; the
- * candidates are a superset of the whole-dictionary call's, so the pick
- * scores at least as high in FP64.  No counterpart in the reference, which
+ * dictionary in memory.  Low-precision modes round block_atoms up to a
+ * multiple of 128; each block is converted to BF16 and its candidates are
+ * rescored against its complex64 rows.  Every candidate the whole-dictionary
+ * call rescores is among the candidates of its block, so the pick scores at
+ * least as high in FP64.  No counterpart in the reference, which
  * always materialises the dictionary (src/mrf.cu:286-321).  epg_ssfp with
  * any number of B1 slices; the other echo types need NATOMS == natoms (they
  * fill one slice). */

@@ -417,6 +417,8 @@ extern "C" int mrf_dict_match_ex(void *stream, const char *echo_type, const floa
         block = ((size_t)1 << 30) / per / 128 * 128;
         if (block < 128) block = 128;
     }
+    // low-precision modes: blocks start on the matcher's 128-atom sub-tiles
+    if (mode != MRF_MATCH_FP32) block = (block + 127) / 128 * 128;
     if (block > natoms) block = natoms;
     const int normalize = (opts && opts->normalize >= 0)
                               ? opts->normalize

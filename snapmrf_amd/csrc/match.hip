@@ -461,7 +461,7 @@ constexpr int kLowpThreads = 4 * 64 * kAtomHalves;
 constexpr int LVT = 256;          // voxels per work-group tile (64 per wave)
 // LK candidates per lane and voxel (4, or 6 for BF16_TOP32); a voxel's four
 // lane lists make its 4*LK candidates per atom chunk.
-constexpr int kMaxChunks = 32;    // rescore merges up to 512 entries per voxel
+constexpr int kMaxChunks = 32;    // atom chunks per voxel tile (rescore merges their lists)
 constexpr int kMinChunkAtoms = 4 * LAT;
 constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
 
@@ -474,7 +474,6 @@ struct LowpArgs {
     float *part_s;               // [chunks][V][4 * LK] scores (-1 = empty)
     uint32_t *part_i;            // [chunks][V][4 * LK] global atom indices
     size_t natoms;               // atoms per slice in this launch
-    size_t chunk;                // atoms per work-group
     size_t row_stride;           // dictionary rows per slice in ahi / alo
     size_t gidx_stride;          // global atom index step per slice
     size_t gidx_base;            // global index of atom 0 of slice 0
@@ -482,6 +481,9 @@ struct LowpArgs {
     int ld;                      // row stride (words)
     int nk;                      // K chunks, ceil(T / KC); nk * KC <= ld
     int nch;                     // atom chunks per tile
+    int nsl;                     // slices every tile meets (1: the tile's own slice)
+    int phase;                   // first sub-tile's dictionary sub-tile index, mod nch
+    int kstep;                   // dictionary sub-tiles per slice, mod nch
 };
 
 // (re, im) -> (im, -re) on a bf16x2 word: the Y operand of Im S.
@@ -581,11 +583,18 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
     const int chunk_id = (int)(L % (unsigned)a.nch);
     if (tile_id >= *a.ntiles) return;
     const Tile tile = a.tiles[tile_id];
-    const size_t a_begin = (size_t)chunk_id * a.chunk;
-    if (a_begin >= a.natoms) return;
-    const size_t a_end = (a_begin + a.chunk < a.natoms) ? a_begin + a.chunk : a.natoms;
-    const size_t row_base = (size_t)tile.slice * a.row_stride;
-    const size_t idx_base = a.gidx_base + (size_t)tile.slice * a.gidx_stride;
+    // The dictionary's sub-tiles (128 atoms of one slice, numbered slice
+    // by slice) go to the chunks of a voxel tile round-robin: chunk c takes
+    // sub-tiles j = c (mod nch).  Neighbouring grid points (near-equal
+    // scores) then fall in different chunks and compete in different lane
+    // lists, and the work-groups of one tile still read every sub-tile once
+    // between them.  A launch over a block of the dictionary (atoms a0 ..
+    // a0 + natoms - 1 of each slice, a0 a multiple of 128) numbers its
+    // sub-tiles as the whole dictionary does (phase, kstep), so each of its
+    // lane lists sees a subset of one lane list of the whole-dictionary
+    // launch.  A chunk with no sub-tile writes empty lists.
+    const size_t a_end = a.natoms;
+    const size_t a_step = (size_t)a.nch * LAT;
     const size_t ldb = (size_t)a.ld * 4;    // row stride in bytes
 
     __shared__ __attribute__((aligned(16))) uint32_t lA[NB * BA];
@@ -620,7 +629,20 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 #pragma unroll
     for (int p = 0; p < PA; p++) aoff[p] = (uint32_t)(row_atom(st_row + RPP * p) * ldb) + st_b;
 
-    for (size_t sub = a_begin; sub < a_end; sub += LAT) {
+    int ks = 0;                             // slice of the launch
+    int r0 = chunk_id - a.phase;            // sub-tile residue of this chunk in slice ks
+    if (r0 < 0) r0 += a.nch;
+    size_t sub = (size_t)r0 * LAT;
+    while (true) {
+        if (sub >= a_end) {
+            if (++ks >= a.nsl) break;
+            r0 -= a.kstep;
+            if (r0 < 0) r0 += a.nch;
+            sub = (size_t)r0 * LAT;
+            continue;
+        }
+        const size_t row_base = (size_t)(tile.slice + ks) * a.row_stride;
+        const size_t idx_base = a.gidx_base + (size_t)(tile.slice + ks) * a.gidx_stride;
         const size_t nsub = (a_end - sub < (size_t)LAT) ? a_end - sub : (size_t)LAT;
         const size_t arow0 = (row_base + sub) * a.ld;
         const __amdgpu_buffer_rsrc_t rA = make_rsrc(a.ahi + arow0, nsub * ldb);
@@ -790,6 +812,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
                 }
             }
         }
+        sub += a_step;
     }
     __syncthreads();
     // the 4 lane lists of each voxel are its 4 * LK candidates of this chunk
@@ -927,7 +950,9 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
     // loads a valid row (slots past the nc candidates re-read candidate 0's
     // row and are ignored), so the loop body has no branches and a group's
     // row loads are all in flight together.
-    constexpr int G = K < 16 ? K : 16;
+    // FP64 sums of 64 candidates in groups of 8: groups of 16 spill at the
+    // two waves per SIMD this kernel is bounded for
+    constexpr int G = K < 16 ? K : ((K > 32 && sizeof(Acc) == 8) ? 8 : 16);
     Acc bs = -1.0f;
     uint32_t bi = 0u;
     const float2 *x = sig + vox * (size_t)T;
@@ -1055,7 +1080,7 @@ int lane_k(int mode)
 {
     switch (mode) {
     case MRF_MATCH_BF16_TOP32: return 6;
-    case MRF_MATCH_BF16_TOP64: return 8;
+    case MRF_MATCH_BF16_TOP64: return 3;
     case MRF_MATCH_SPLIT2: return 4;
     case MRF_MATCH_SPLIT3: return 2;
     default: return 3;
@@ -1064,19 +1089,23 @@ int lane_k(int mode)
 
 
 
-// atom chunks per voxel tile: about 2048 work-groups, 8 to kMaxChunks
-size_t lowp_want(size_t max_tiles)
+// atom chunks per voxel tile: about 2048 work-groups, 8 to kMaxChunks.
+// BF16_TOP64 always takes kMaxChunks: its lane lists are the short 3-entry
+// ones of TOP8/TOP16, and lists that each cover 1/kMaxChunks of the atoms
+// keep the FP32 argmax among its 64 rescored candidates.
+size_t lowp_want(size_t max_tiles, int mode)
 {
+    if (mode == MRF_MATCH_BF16_TOP64) return kMaxChunks;
     size_t want = (2048 + max_tiles - 1) / max_tiles;
     if (want < 8) want = 8;
     if (want > (size_t)kMaxChunks) want = kMaxChunks;
     return want;
 }
 
-// chunking of nat atoms; at most lowp_want(max_tiles) chunks for any nat
-void lowp_chunks(size_t nat, size_t max_tiles, size_t *chunk, size_t *nchunks)
+// chunking of nat atoms; at most lowp_want chunks for any nat
+void lowp_chunks(size_t nat, size_t max_tiles, int mode, size_t *chunk, size_t *nchunks)
 {
-    const size_t want = lowp_want(max_tiles);
+    const size_t want = lowp_want(max_tiles, mode);
     size_t c = (nat + want - 1) / want;
     if (c < (size_t)kMinChunkAtoms) c = kMinChunkAtoms;
     c = (c + LAT - 1) / LAT * LAT;
@@ -1108,7 +1137,7 @@ Scratch carve(void *base, const MatchLaunch &l)
     s.tiles = (Tile *)p;
     p += align256(s.max_tiles * sizeof(Tile));
     if (l.mode != MRF_MATCH_FP32) {
-        lowp_chunks(nat, s.max_tiles, &s.chunk, &s.nchunks);
+        lowp_chunks(nat, s.max_tiles, l.mode, &s.chunk, &s.nchunks);
         const size_t ld = lowp_ld(l.T);
         s.shi = (uint32_t *)p;
         p += align256(l.V * ld * 4);
@@ -1117,7 +1146,7 @@ Scratch carve(void *base, const MatchLaunch &l)
             p += align256(l.V * ld * 4);
         }
         // candidate lists for up to lowp_want chunks (any launch's nat)
-        s.part_chunks = lowp_want(s.max_tiles);
+        s.part_chunks = lowp_want(s.max_tiles, l.mode);
         s.part_s = (float *)p;
         const size_t ck = 4 * (size_t)kAtomHalves * lane_k(l.mode);
         p += align256(s.part_chunks * l.V * ck * 4);
@@ -1236,15 +1265,19 @@ RescoreArgs rescore_args(const MatchLaunch &l, const Scratch &sc)
 }
 
 // One low-precision launch (candidates, then their rescore) over atoms
-// [0, nat) of every slice of hi / lo (rows row_stride apart per slice),
-// global index gbase + slice * gstride + atom.
+// [0, nat) of nsl slices of hi / lo (rows row_stride apart per slice; nsl = 1:
+// each voxel tile's own slice), global index gbase + slice * gstride + atom.
+// The launch's atom 0 is atom a0 = gbase mod gstride of its slice; its
+// sub-tiles take the chunks of the whole dictionary's sub-tiles.
 int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, const uint32_t *lo,
-                size_t nat, size_t row_stride, size_t gstride, size_t gbase, RescoreArgs r,
-                hipStream_t s)
+                size_t nat, int nsl, size_t row_stride, size_t gstride, size_t gbase,
+                RescoreArgs r, hipStream_t s)
 {
-    size_t chunk = 0, nchunks = 0;
-    lowp_chunks(nat, sc.max_tiles, &chunk, &nchunks);
+    const size_t nchunks = sc.nchunks;
     if (nchunks > sc.part_chunks) return mrf_fail(MRF_E_ARG, "candidate storage too small");
+    const size_t a0 = gbase % gstride;
+    if (a0 % LAT) return mrf_fail(MRF_E_ARG, "block start not a multiple of %d atoms", LAT);
+    const size_t nsub_w = (l.natoms + LAT - 1) / LAT;   // sub-tiles per dictionary slice
     LowpArgs b{};
     b.ahi = hi;
     b.alo = lo;
@@ -1256,13 +1289,15 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     b.part_s = sc.part_s;
     b.part_i = sc.part_i;
     b.natoms = nat;
-    b.chunk = chunk;
     b.row_stride = row_stride;
     b.gidx_stride = gstride;
     b.gidx_base = gbase;
     b.V = l.V;
     b.ld = (int)lowp_ld(l.T);
     b.nch = (int)nchunks;
+    b.nsl = nsl;
+    b.phase = (int)((a0 / LAT) % nchunks);
+    b.kstep = (int)(nsub_w % nchunks);
     const size_t nwg = (sc.max_tiles * nchunks + 7) / 8 * 8;
     if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
     const dim3 grid((unsigned)nwg);
@@ -1274,10 +1309,11 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     // TOP8/TOP16: 32-timepoint chunks, 3-entry lists, two buffers.  TOP32:
     // 6-entry lists (3-entry ones missed the exact pick on 0.11 % of voxels,
     // 6-entry ones on 0.005 %) fit 32-timepoint chunks with one staging
-    // buffer (2 % faster than two buffers of 16).  TOP64: 8-entry lists,
-    // 16-timepoint chunks, one buffer (161,792 bytes of LDS); on the
-    // 871k-atom grid (B0 steps of 2 Hz) it agrees with FP32 on 99.92 % of
-    // voxels where TOP32 reaches 99.57 %.  SPLIT2: 16, 4-entry.
+    // buffer (2 % faster than two buffers of 16).  TOP64: the TOP8/TOP16
+    // kernel over kMaxChunks strided chunks (32 x 24 candidates per voxel,
+    // top 64 rescored); on 500,000 atoms of the 871k-atom grid (B0 steps of
+    // 2 Hz) it agrees with FP32 on 99.92 % of voxels in 358 ms, where 8-entry
+    // lists of 16-timepoint chunks took 421 ms.  SPLIT2: 16, 4-entry.
     // SPLIT3: 16, 2-entry (their fragments of 32-timepoint chunks do not fit
     // the registers of two waves per SIMD).
     const dim3 rg((unsigned)((l.V + 3) / 4));
@@ -1296,9 +1332,9 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
         match_lowp_kernel<2, 16, 4, 2><<<grid, blk, 0, s>>>(b);
         MRF_RESCORE(8, 32);
     } else if (l.mode == MRF_MATCH_BF16_TOP64) {
-        b.nk = (l.T + 15) / 16;
-        match_lowp_kernel<1, 16, 8, 1><<<grid, blk, 0, s>>>(b);
-        MRF_RESCORE(64, 64);
+        b.nk = (l.T + 31) / 32;
+        match_lowp_kernel<1, 32, 3, 2><<<grid, blk, 0, s>>>(b);
+        MRF_RESCORE(64, 24);
     } else if (l.mode == MRF_MATCH_BF16_TOP32) {
         b.nk = (l.T + 31) / 32;
         match_lowp_kernel<1, 32, 6, 1><<<grid, blk, 0, s>>>(b);
@@ -1326,7 +1362,9 @@ int launch_match(const MatchLaunch &l, void *scratch, hipStream_t s)
         if (int rc = lowp_prepare_signals(l, sc, s)) return rc;
         RescoreArgs r = rescore_args(l, sc);
         r.atoms = l.atoms;
-        if (int rc = launch_lowp(l, sc, l.lowp_hi, l.lowp_lo, nat, nat, nat, 0, r, s)) return rc;
+        if (int rc = launch_lowp(l, sc, l.lowp_hi, l.lowp_lo, l.natoms,
+                                 l.vox_slice ? 1 : l.n_slices, l.natoms, l.natoms, 0, r, s))
+            return rc;
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "match launch: %s", hipGetErrorString(e));
     }
@@ -1389,16 +1427,9 @@ int launch_match_blocks(const MatchLaunch &l, size_t block, float2 *buf,
         r.nb = nb;
         r.best_s = lb->best_s;
         r.best_i = lb->best_i;
-        if (l.vox_slice) {
-            if (int rc = launch_lowp(l, sc, lb->hi, lb->lo, nb, nb, l.natoms, a0, r, s)) return rc;
-        } else {
-            for (int k = 0; k < l.n_slices; k++) {
-                const size_t off = (size_t)k * nb * ld;
-                if (int rc = launch_lowp(l, sc, lb->hi + off, need_lo ? lb->lo + off : nullptr, nb,
-                                         0, 0, (size_t)k * l.natoms + a0, r, s))
-                    return rc;
-            }
-        }
+        if (int rc = launch_lowp(l, sc, lb->hi, need_lo ? lb->lo : nullptr, nb,
+                                 l.vox_slice ? 1 : l.n_slices, nb, l.natoms, a0, r, s))
+            return rc;
     }
     if (!lowp) maps_kernel<<<vb, 256, 0, s>>>(sc.keys, l.params, l.V, l.maps, l.idx);
     hipError_t e = hipGetLastError();

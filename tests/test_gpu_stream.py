@@ -102,8 +102,10 @@ def _fp64_score(atoms, sig, idx):
 def test_streamed_low_precision(gpu, use_b1):
     """Low-precision modes streamed block by block: each block's candidates
     are rescored against its own complex64 rows and merged into a running
-    best, a superset of the whole-dictionary call's candidates, so on every
-    voxel the streamed pick's FP64 score is at least the whole call's."""
+    best.  A block's sub-tiles keep the atom chunks and lane lists they have
+    in the whole-dictionary call, so every candidate that call rescores is
+    among its block's, and on every voxel the streamed pick's FP64 score is
+    at least the whole call's."""
     import oracle
     raw, params, d_params, A, natoms, d_atoms, sig, d_b1, ns = _inputs(gpu, V=2000)
     b1 = d_b1 if use_b1 else None
@@ -148,9 +150,10 @@ def _streamed_mode(torch, raw, d_params, A, natoms, sig, d_b1, block, mode):
 @pytest.mark.parametrize("T_, V_, block", [(37, 5, 64), (16, 130, 1000), (101, 257, 300)])
 def test_streamed_ragged(gpu, T_, V_, block):
     """Odd and tiny T, few voxels, blocks larger than a slice or not a
-    multiple of the tile: FP32 streamed equals the whole dictionary; the
-    low-precision modes' streamed picks score at least as high in FP64 as the
-    whole-dictionary call's."""
+    multiple of the tile (the low-precision modes round them up to 128
+    atoms), slices that are not a multiple of 128 atoms: FP32 streamed equals
+    the whole dictionary; the low-precision modes' streamed picks score at
+    least as high in FP64 as the whole-dictionary call's."""
     raw = synth.csv_columns(synth.sequence(T_, seed=9), "epg_ssfp")
     params, A, natoms = sm.build_params("100:100:2000", "20:20:300", "-20:10:20", "0.9:0.1:1.1")
     d_params = gpu.from_numpy(params).cuda()
