@@ -38,7 +38,16 @@ namespace mrf {
 
 namespace {
 
-constexpr int kCkpt = 50;                 // TRs between checkpoints
+constexpr int kCkpt = 50;                 // TRs between checkpoints (ND = 4)
+// The adjoint (ND = 0) checkpoints every kSegLds TRs and keeps a segment's
+// TR-start states in LDS (kSegLds x 3 float2 per lane) rather than in a
+// global segment buffer: one state write and read per TR in LDS instead of
+// 24 bytes per lane and TR to and from HBM, for 50 / kSegLds times the
+// checkpoint traffic.  Measured on 57,600 atoms (T = 1000, w = 30): 2, 3, 5,
+// 8, 10, 16 TRs per segment took 22.4, 21.8, 19.1, 22.1, 21.6, 38.0 ms
+// (5 is the longest segment at which three work-groups fit a CU's LDS; the
+// 50-TR global segment buffer took 22.2 ms).
+constexpr int kSegLds = 5;
 constexpr int kWaves = 4;                 // atoms per work-group
 constexpr float kCB0 = (float)(2.0 * M_PI / 1000.0);
 constexpr float kTI = 40.0f;              // inversion delay, functions.cu:1242
@@ -701,7 +710,7 @@ struct RevArgs {
     float fa_unit;             // d alpha / d FA_raw
     float wts[4];              // ND = 4
     double *loss;              // ND = 4: [n]
-    float *scratch;            // per atom (nck + kCkpt) * 6 (1 + ND) * w floats
+    float *scratch;            // per atom diff_scratch_floats(ND, T, w) floats
     int seq_lds;               // grad_seq summed per work-group in LDS
 };
 
@@ -746,14 +755,42 @@ __device__ bool inv4(const double *A, double *X)
 
 // One atom's sweeps (the lane group of SUB lanes); gs_lds is the work-group's
 // grad_seq partial sums when A.seq_lds.
-template <int ND, int SUB>
-__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds, TrA *coef_lds)
+// checkpoint interval of a reverse sweep
+template <int ND>
+constexpr int ckpt_of() { return ND == 0 ? kSegLds : kCkpt; }
+
+// a lane's TR-start state in the LDS segment buffer (ND = 0): slot [tr][3]
+// of the wave's 64 lanes, float2 each
+__device__ __forceinline__ void seg_put(float2 *seg, int tr, const Aug<0> &y)
 {
+    const int l = threadIdx.x & 63;
+    seg[(tr * 3 + 0) * 64 + l] = y.fp[0];
+    seg[(tr * 3 + 1) * 64 + l] = y.fm[0];
+    seg[(tr * 3 + 2) * 64 + l] = y.z[0];
+}
+
+__device__ __forceinline__ void seg_get(const float2 *seg, int tr, int w, int k, Aug<0> &y)
+{
+    const int l = threadIdx.x & 63;
+    if (k < w) {
+        y.fp[0] = seg[(tr * 3 + 0) * 64 + l];
+        y.fm[0] = seg[(tr * 3 + 1) * 64 + l];
+        y.z[0] = seg[(tr * 3 + 2) * 64 + l];
+    } else {
+        y.fp[0] = y.fm[0] = y.z[0] = {0.f, 0.f};
+    }
+}
+
+template <int ND, int SUB>
+__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds, TrA *coef_lds,
+                                         float2 *seg_lds)
+{
+    constexpr int CKPT = ckpt_of<ND>();
     const int k = threadIdx.x & (SUB - 1);
     const int w = A.w, T = A.T;
     const int C = 6 * (1 + ND) * w;
-    float *ck = A.scratch + a * (size_t)(A.nck + kCkpt) * C;
-    float *seg = ck + (size_t)A.nck * C;
+    float *ck = A.scratch + a * (size_t)(A.nck + (ND == 0 ? 0 : kCkpt)) * C;
+    float *seg = ck + (size_t)A.nck * C;      // ND = 4: global segment buffer
     const Atom p = load_atom(A.theta, a);
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
 
@@ -766,7 +803,7 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
 #pragma unroll
     for (int i = 0; i < 10; i++) F[i] = 0.f;
     for (int t = 0; t < T; t++) {
-        if (t % kCkpt == 0) st_store(ck + (size_t)(t / kCkpt) * C, w, k, y);
+        if (t % CKPT == 0) st_store(ck + (size_t)(t / CKPT) * C, w, k, y);
         const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
         if constexpr (ND > 0) {
             Aug<ND> r;
@@ -823,30 +860,45 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     for (int j = 0; j <= ND; j++) l.fp[j] = l.fm[j] = l.z[j] = {0.f, 0.f};
     float gth[4] = {0.f, 0.f, 0.f, 0.f};
     const float beta = kCB0 * p.B0;
-    static_assert(kCkpt <= 64, "a segment's cotangents live in two registers per lane");
-    for (int sgi = A.nck - 1; sgi >= 0; sgi--) {
-        const int t0 = sgi * kCkpt;
-        const int t1 = min(T, t0 + kCkpt);
-        // the segment's cotangents, loaded while its states are recomputed
-        // and read back per TR by readlane
-        float2 cv0 = {0.f, 0.f}, cv1 = {0.f, 0.f};
+    static_assert(CKPT <= 64, "a segment's cotangents live in two registers per lane");
+    // ND = 0: the checkpoint and cotangents of a segment are loaded while
+    // the segment after it (in time) is reversed
+    Aug<ND> cknext;
+    float2 cvn0 = {0.f, 0.f}, cvn1 = {0.f, 0.f};
+    auto seg_inputs = [&](int sgi, Aug<ND> &ckp, float2 &c0, float2 &c1) {
+        const int t0 = sgi * CKPT, t1 = min(T, t0 + CKPT);
+        st_load(ck + (size_t)sgi * C, w, k, ckp);
+        c0 = c1 = {0.f, 0.f};
         if constexpr (ND == 0) {
             const float2 *cr = A.cot + a * T;
-            if (t0 + k < t1) cv0 = cr[t0 + k];
-            if (SUB == 32 && t0 + 32 + k < t1) cv1 = cr[t0 + 32 + k];
+            if (t0 + k < t1) c0 = cr[t0 + k];
+            if (SUB == 32 && t0 + 32 + k < t1) c1 = cr[t0 + 32 + k];
         }
-        st_load(ck + (size_t)sgi * C, w, k, y);
+    };
+    seg_inputs(A.nck - 1, cknext, cvn0, cvn1);
+    for (int sgi = A.nck - 1; sgi >= 0; sgi--) {
+        const int t0 = sgi * CKPT;
+        const int t1 = min(T, t0 + CKPT);
+        // the segment's cotangents, read back per TR by readlane
+        const float2 cv0 = cvn0, cv1 = cvn1;
+        y = cknext;
+        if (sgi > 0) seg_inputs(sgi - 1, cknext, cvn0, cvn1);
         for (int t = t0; t < t1; t++) {
-            st_store(seg + (size_t)(t - t0) * C, w, k, y);
+            if constexpr (ND == 0) seg_put(seg_lds, t - t0, y);
+            else st_store(seg + (size_t)(t - t0) * C, w, k, y);
             f_tr<ND, SUB>(y, coef<SUB>(cc, A.seq, T, p, t, k), p, k, w, nullptr);
         }
         // the TR-start state of the next (earlier) TR is loaded while this
         // TR is reversed
         Aug<ND> ynext;
-        st_load(seg + (size_t)(t1 - 1 - t0) * C, w, k, ynext);
+        if constexpr (ND == 0) seg_get(seg_lds, t1 - 1 - t0, w, k, ynext);
+        else st_load(seg + (size_t)(t1 - 1 - t0) * C, w, k, ynext);
         for (int t = t1 - 1; t >= t0; t--) {
             const Aug<ND> yn = ynext;
-            if (t > t0) st_load(seg + (size_t)(t - 1 - t0) * C, w, k, ynext);
+            if (t > t0) {
+                if constexpr (ND == 0) seg_get(seg_lds, t - 1 - t0, w, k, ynext);
+                else st_load(seg + (size_t)(t - 1 - t0) * C, w, k, ynext);
+            }
             const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
             float ga = 0.f, gt = 0.f;
             // the TR's forward chain once: x1 = rf(yn), x2 = relax_a(x1),
@@ -936,18 +988,21 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
 // grad_seq of the atoms of a work-group is summed in LDS (2T floats of
 // dynamic LDS when A.seq_lds): one LDS add per atom and TR, then one global
 // atomic per entry and work-group instead of one per atom.
-// ND = 0 fits 128 VGPRs: four waves per SIMD (6 % faster than three)
+// ND = 0: the LDS (segment states, coefficient cache, grad_seq sums; about
+// 50 KB) holds three work-groups per CU, three waves per SIMD
 template <int ND, int SUB>
-__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 4 : 1) void rev_kernel(RevArgs A)
+__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 3 : 1) void rev_kernel(RevArgs A)
 {
     extern __shared__ float gs_lds[];
     __shared__ TrA coef_lds[kCoefLds];
+    __shared__ float2 seg_lds[ND == 0 ? kWaves * kSegLds * 3 * 64 : 1];
     if (A.seq_lds) {
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) gs_lds[i] = 0.f;
         __syncthreads();
     }
     const size_t a = group_atom<SUB>();
-    if (a < A.n) rev_atom<ND, SUB>(A, a, gs_lds, coef_lds);
+    if (a < A.n)
+        rev_atom<ND, SUB>(A, a, gs_lds, coef_lds, seg_lds + (threadIdx.x >> 6) * kSegLds * 3 * 64);
     if (A.seq_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) {
@@ -1256,6 +1311,7 @@ dim3 groups(size_t n, int sub)
 
 size_t diff_scratch_floats(int nd, int T, int w)
 {
+    if (nd == 0) return (size_t)((T + kSegLds - 1) / kSegLds) * 6 * w;
     const int nck = (T + kCkpt - 1) / kCkpt;
     return (size_t)(nck + kCkpt) * 6 * (1 + nd) * w;
 }
@@ -1280,7 +1336,7 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.seq = seq;
     A.T = T;
     A.w = w;
-    A.nck = (T + kCkpt - 1) / kCkpt;
+    A.nck = (T + kSegLds - 1) / kSegLds;
     A.theta = theta;
     A.n = n;
     A.cot = cot;
