@@ -255,12 +255,11 @@ __device__ __forceinline__ void wave_sync()
 // group reads each TR's entry back, so a lane does 1 / SUB of that work.  The
 // two groups of a wave walk the same TRs, so a block changes for the whole
 // wave at once; a group's entries are written and read by that wave only.
-struct alignas(16) TrA {
+struct alignas(8) TrA {
     float sa, ca, e1a, e2a, e1b, e2b;
     float2 pa, pb;
-    float pad[2];
 };
-static_assert(sizeof(TrA) == 48, "TrA layout");
+static_assert(sizeof(TrA) == 40, "TrA layout");
 
 constexpr int kCoefLds = kWaves * 64;   // TrA entries per work-group
 
@@ -294,7 +293,6 @@ __device__ __forceinline__ TrK coef(CoefCache<SUB> &cc, const DTr *seq, int T, c
             e.e2b = c.e2b;
             e.pa = c.pa;
             e.pb = c.pb;
-            e.pad[0] = e.pad[1] = 0.f;
             cc.blk[k] = e;
         }
         cc.cur = b;
@@ -712,6 +710,7 @@ struct RevArgs {
     double *loss;              // ND = 4: [n]
     float *scratch;            // per atom diff_scratch_floats(ND, T, w) floats
     int seq_lds;               // grad_seq summed per work-group in LDS
+    float *seq_part;           // or per atom, [n][2][T], summed by seq_reduce_kernel
 };
 
 // grad_seq partial sums go through LDS up to this T (2T floats)
@@ -719,7 +718,7 @@ constexpr int kSeqLdsMaxT = 8192;
 
 size_t rev_lds_bytes(RevArgs &A)
 {
-    A.seq_lds = A.grad_seq != nullptr && A.T <= kSeqLdsMaxT;
+    A.seq_lds = A.grad_seq != nullptr && !A.seq_part && A.T <= kSeqLdsMaxT;
     return A.seq_lds ? (size_t)2 * A.T * sizeof(float) : 0;
 }
 
@@ -954,7 +953,10 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
                 ga = group_sum<SUB>(ga);
                 gt = group_sum<SUB>(gt);
                 if (k == 0) {
-                    if (A.seq_lds) {
+                    if (A.seq_part) {
+                        A.seq_part[a * 2 * T + t] = ga * A.fa_unit;
+                        A.seq_part[a * 2 * T + T + t] = gt;
+                    } else if (A.seq_lds) {
                         atomicAdd(&gs_lds[t], ga * A.fa_unit);
                         atomicAdd(&gs_lds[T + t], gt);
                     } else {
@@ -988,10 +990,11 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
 // grad_seq of the atoms of a work-group is summed in LDS (2T floats of
 // dynamic LDS when A.seq_lds): one LDS add per atom and TR, then one global
 // atomic per entry and work-group instead of one per atom.
-// ND = 0: the LDS (segment states, coefficient cache, grad_seq sums; about
-// 50 KB) holds three work-groups per CU, three waves per SIMD
+// ND = 0: the segment states and the coefficient cache (40 KB; grad_seq
+// goes out per atom, A.seq_part) hold four work-groups per CU, four waves
+// per SIMD at 128 VGPRs
 template <int ND, int SUB>
-__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 3 : 1) void rev_kernel(RevArgs A)
+__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 4 : 1) void rev_kernel(RevArgs A)
 {
     extern __shared__ float gs_lds[];
     __shared__ TrA coef_lds[kCoefLds];
@@ -1010,6 +1013,18 @@ __global__ __launch_bounds__(64 * kWaves, ND == 0 ? 3 : 1) void rev_kernel(RevAr
             if (v != 0.f) atomicAdd(&A.grad_seq[i], v);
         }
     }
+}
+
+// grad_seq[i] += sum over atoms of part[a][i], i < T2: block (x, y) sums
+// entries 256 x .. 256 x + 255 over atoms y, y + gridDim.y, ...
+__global__ __launch_bounds__(256) void seq_reduce_kernel(const float *part, size_t n, int T2,
+                                                         float *grad_seq)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= T2) return;
+    float v = 0.f;
+    for (size_t a = blockIdx.y; a < n; a += gridDim.y) v += part[a * T2 + i];
+    atomicAdd(&grad_seq[i], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -1311,7 +1326,8 @@ dim3 groups(size_t n, int sub)
 
 size_t diff_scratch_floats(int nd, int T, int w)
 {
-    if (nd == 0) return (size_t)((T + kSegLds - 1) / kSegLds) * 6 * w;
+    // ND = 0: checkpoints, then the atom's grad_seq partials (2T)
+    if (nd == 0) return (size_t)((T + kSegLds - 1) / kSegLds) * 6 * w + (size_t)2 * T;
     const int nck = (T + kCkpt - 1) / kCkpt;
     return (size_t)(nck + kCkpt) * 6 * (1 + nd) * w;
 }
@@ -1344,9 +1360,17 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.grad_seq = grad_seq;
     A.fa_unit = fa_unit;
     A.scratch = scratch;
+    // the checkpoints of all n atoms, then their grad_seq partials
+    const size_t ck_floats = (size_t)A.nck * 6 * w;
+    A.seq_part = grad_seq ? scratch + n * ck_floats : nullptr;
     const size_t lds = rev_lds_bytes(A);
     if (w <= 32) rev_kernel<0, 32><<<groups(n, 32), 64 * kWaves, lds, s>>>(A);
     else rev_kernel<0, 64><<<groups(n, 64), 64 * kWaves, lds, s>>>(A);
+    if (grad_seq) {
+        const unsigned gy = (unsigned)(n < 64 ? n : 64);
+        seq_reduce_kernel<<<dim3((unsigned)((2 * T + 255) / 256), gy), 256, 0, s>>>(
+            A.seq_part, n, 2 * T, grad_seq);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "adjoint launch: %s", hipGetErrorString(e));
 }
