@@ -247,6 +247,7 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
                             "peak": PEAK_BF16, "unit": "TFLOP/s",
                             "frac": res[best]["tflops_algorithmic"] / PEAK_BF16,
                             "traffic": measured_traffic("match_" + best),
+                            "mfma_busy": measured_counters("match_" + best, "mfma"),
                             "kernel": kernels[best], "mode": best}
     # FP64 oracle on a seeded subset: T1/T2/B0/PD errors of every path, and
     # on a sample of the voxels where a low-precision pick differs from the
@@ -470,7 +471,10 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
         "adjoint": {"ms": adj_ms, "atoms_per_s": V / (adj_ms * 1e-3),
                     "tflops_algorithmic": af / (adj_ms * 1e-3) / 1e12,
                     "frac_fp32_peak": af / (adj_ms * 1e-3) / 1e12 / PEAK_FP32,
-                    "flop_per_state_tr": FLOP_ADJ, "checkpoint_every": 50},
+                    "flop_per_state_tr": FLOP_ADJ, "checkpoint_every": 5,
+                    "segment_states": "LDS",
+                    "traffic_per_call": measured_traffic("adjoint"),
+                    "valu_issue": measured_counters("adjoint", "valu")},
         "gauss_newton": {"ms": gn_ms, "iters": iters, "voxels_per_s": V / (gn_ms * 1e-3),
                          "start_from": "FP32 grid match (all B1 slices)",
                          "errors_grid_match": errs(start), "errors_refined": errs(ref),
@@ -688,6 +692,22 @@ def measured_traffic(key: str):
         return None
 
 
+def measured_counters(key: str, group: str):
+    """A kernel's VALU or MFMA counter summary from profiles/traffic.json
+    (rocprofv3 --pmc over bench.py --profile), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            c = json.load(f)[key][group]
+    except (OSError, KeyError, ValueError):
+        return None
+    if group == "valu":
+        return {"valu_insts_per_launch": c["valu_insts"],
+                "insts_per_cycle_per_simd": c["ipc_per_simd"],
+                "dual_issued_frac": c["dual_issued_frac"],
+                "issue_ceiling_per_cycle_per_simd": 0.5}
+    return {"mfma_busy_frac": c["mfma_busy_frac"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -780,6 +800,10 @@ def main() -> int:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": measured_traffic("epg"),
                      "kernel": "epg_fused_kernel<30,false,true,false>",
+                     # the S/D form executes 24 of the 102 algorithmic flop per
+                     # state and TR as VALU instructions; how busy the SIMDs are
+                     # is in the VALU issue counters
+                     "valu_issue": measured_counters("epg", "valu"),
                      "algorithmic_flops_per_launch": flops, "ms_per_launch": kern_ms},
         "ref_published_atoms_per_s": 10765,
     }
@@ -821,7 +845,8 @@ def main() -> int:
                                  # three real products per complex product:
                                  # the MFMA pipes issue 6 of the 8 flops
                                  "mfma_flops_per_launch": 0.75 * mflops,
-                                 "frac_mfma_issued": 0.75 * ach_m / PEAK_FP32}
+                                 "frac_mfma_issued": 0.75 * ach_m / PEAK_FP32,
+                                 "mfma_busy": measured_counters("match", "mfma")}
         if not args.skip_lowp:
             out["precision_c2"] = precision_check(
                 sm, torch, np, d_atoms, d_params, params, T, A, natoms, d_sig, V, d_b1, match_ms,
