@@ -7,7 +7,10 @@
 //     VGPRs.  The gradient shift crosses lanes only at the group-internal
 //     boundaries, with one DPP row_shr / row_shl per value.
 //   * The TR loop runs inside the kernel; the per-TR, per-B1-slice RF
-//     coefficients come from a 64-byte TrCoef record read with uniform
+//     coefficients come from a 64-byte TrCoef record.  One-lane-per-atom
+//     waves read each TR's factors from an LDS table that the wave's lanes
+//     fill together for each group of atoms sharing (T2, B0); multi-lane
+//     groups form them per lane from TrCoef records read with uniform
 //     (scalar) loads.
 //   * Output atoms are [slice][atom][t] complex64 (RA dims (T, A), time
 //     fastest), the reference's layout (src/functions.cu:609-617).
@@ -188,29 +191,22 @@ struct FusedCtx {
     int j;            // lane within the atom's group
 };
 
-// One TR of the fused frame-form step with the F- array offset R; returns
-// the sample E2(TE) e^{i theta_TE} e^{i phi_n} P+_0.
-//
-// The zero-phase RF is evaluated through S = A + B and D = A - B (A, B, C =
-// F~+, F~-, Z of a state): since (1+cos a)/2 + (1-cos a)/2 = 1 and their
-// difference is cos a,
-//     P+ = (S + W)/2,  P- = (S - W)/2,  W = cos(a) D + 2 sin(a) (C.y, -C.x),
-//     Q  = cos(a) C + sin(a)/2 (D.y, -D.x),
-// and the exact factor 1/2 is folded into H and into the sample factor.  That
-// is 24 operations per state (20 with a real H), 10 of them additions.
-template <int M, int R, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
-__device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
-                                             const FusedCtx &x, const EpgArgs &a)
+// Factors of one TR that depend on the atom's T2 and B0 but not on T1:
+// (H.x, H.y, csave.x, csave.y) with
+//     H     = E2(TE) E2(TR-TE) e^{i(theta_TE + theta_TR-TE)} e^{i(phi_n - phi_{n+1})} / 2,
+//     csave = E2(TE) e^{i theta_TE} e^{i phi_n} / 2
+// (k2 = -log2(e)/T2, b0r = B0 in revolutions per ms).
+template <bool PHASE, bool HREAL>
+__device__ __forceinline__ float4 group_factors(const TrCoef &c, float k2, double b0r)
 {
-    const float ea = decay(c.te, x.k2);
-    const float eb = decay(c.tmte, x.k2);
-    const float E1 = decay(c.tr, x.k1);
+    const float ea = decay(c.te, k2);
+    const float eb = decay(c.tmte, k2);
     float2 H, csave;
     if (PHASE) {
         // B0 precession over TE and TR-TE (src/functions.cu:585-607)
         float sa_, ca_, sb_, cb_;
-        sincos_rev(x.b0r * (double)c.te, &sa_, &ca_);
-        sincos_rev(x.b0r * (double)c.tmte, &sb_, &cb_);
+        sincos_rev(b0r * (double)c.te, &sa_, &ca_);
+        sincos_rev(b0r * (double)c.tmte, &sb_, &cb_);
         const float2 e = cmul(make_float2(ca_, sa_), make_float2(cb_, sb_));
         const float g = 0.5f * ea * eb;
         const float2 er = cmul(e, make_float2(c.rr, c.ri));
@@ -224,8 +220,36 @@ __device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
         const float h = 0.5f * ea;
         csave = make_float2(h * c.zr, h * c.zi);
     }
-    const float ca = c.ca, s2 = 2.0f * c.sa;
-    const float eca = E1 * c.ca, eh = E1 * 0.5f * c.sa;
+    return make_float4(H.x, H.y, csave.x, csave.y);
+}
+
+// Factors of one TR that depend on the B1 slice only: (cos a, 2 sin a,
+// sin(a)/2, TR).
+__device__ __forceinline__ float4 slice_factors(const TrCoef &c)
+{
+    return make_float4(c.ca, 2.0f * c.sa, 0.5f * c.sa, c.tr);
+}
+
+// One TR of the fused frame-form step with the F- array offset R, given the
+// group factors gf (group_factors) and the slice factors sf (slice_factors);
+// returns the sample E2(TE) e^{i theta_TE} e^{i phi_n} P+_0.
+//
+// The zero-phase RF is evaluated through S = A + B and D = A - B (A, B, C =
+// F~+, F~-, Z of a state): since (1+cos a)/2 + (1-cos a)/2 = 1 and their
+// difference is cos a,
+//     P+ = (S + W)/2,  P- = (S - W)/2,  W = cos(a) D + 2 sin(a) (C.y, -C.x),
+//     Q  = cos(a) C + sin(a)/2 (D.y, -D.x),
+// and the exact factor 1/2 is folded into H and into the sample factor.  That
+// is 24 operations per state (20 with a real H), 10 of them additions.
+template <int M, int R, bool MULTI, bool HREAL, bool TRUNC>
+__device__ __forceinline__ float2 fused_core(FusedState<M> &s, float4 gf, float4 sf,
+                                             const FusedCtx &x, const EpgArgs &a)
+{
+    const float2 H = make_float2(gf.x, HREAL ? 0.f : gf.y);
+    const float2 csave = make_float2(gf.z, gf.w);
+    const float E1 = decay(sf.w, x.k1);
+    const float ca = sf.x, s2 = sf.y;
+    const float eca = E1 * sf.x, eh = E1 * sf.z;
 
     float2 top = make_float2(0.f, 0.f), d0 = top, fp0 = top, u0 = top;
 #pragma unroll
@@ -263,6 +287,15 @@ __device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
             if (i == a.istar && x.j == a.jstar) s.fm[i + R + 1] = make_float2(0.f, 0.f);
     }
     return cmul(csave, u0);
+}
+
+// One TR with the factors formed by the lane itself.
+template <int M, int R, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
+__device__ __forceinline__ float2 fused_step(FusedState<M> &s, const TrCoef &c,
+                                             const FusedCtx &x, const EpgArgs &a)
+{
+    return fused_core<M, R, MULTI, HREAL, TRUNC>(s, group_factors<PHASE, HREAL>(c, x.k2, x.b0r),
+                                                 slice_factors(c), x, a);
 }
 
 template <int M, int SH>
@@ -358,6 +391,77 @@ __device__ __forceinline__ void normalize_rows(const WaveStage &w, int per_row)
     }
 }
 
+// Per-TR factor tables of one-lane-per-atom waves.  The per-TR factors that
+// do not depend on T1 (two exp2, two sincos with their FP64 range reduction,
+// the complex products of group_factors) are the same for every atom of a
+// wave with the same (T2, B0): in the reference's atom order (T1 fastest) a
+// 64-atom wave holds one or two such groups.  The wave finds its G groups,
+// and its lanes form the group factors of a chunk of TRs together (G x chunk
+// entries, chunk = 64 TRs for G <= 4, fewer for more groups, so a chunk never
+// has more than 256 entries), together with the slice factors of the chunk's
+// TRs; the TR loop then reads each TR's factors from LDS with two broadcast
+// loads instead of forming them per lane.  Two buffers: the next chunk is
+// filled while the current one is read.
+constexpr int kTabGroupEntries = 256;
+constexpr int kTabChunkMax = 64;
+
+struct TabBuf {
+    float4 grp[kTabGroupEntries];   // [group][TR of the chunk] group factors
+    float4 trc[kTabChunkMax];       // [TR of the chunk] slice factors
+};
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Groups of equal (T2, B0) bit patterns among the 64 lanes of a wave.
+// Returns G (1..64); *my_g is the lane's group and lane g < G holds in
+// *lead the lowest lane of group g.
+__device__ __forceinline__ int wave_groups(float T2, float B0, int lane, int *my_g, int *lead)
+{
+    const int kt = __float_as_int(T2), kb = __float_as_int(B0);
+    unsigned long long todo = ~0ull;
+    int G = 0;
+    *my_g = 0;
+    *lead = 0;
+    while (todo) {
+        // the lowest unassigned lane founds a group; a lane that matches it
+        // is unassigned, since earlier groups hold other keys
+        const int ld = __builtin_ctzll(todo);
+        const int lt = __shfl(kt, ld), lb = __shfl(kb, ld);
+        const bool same = kt == lt && kb == lb;
+        const unsigned long long m = __ballot(same);
+        if (same) *my_g = G;
+        if (lane == G) *lead = ld;
+        todo &= ~m;
+        G++;
+    }
+    return G;
+}
+
+// Fills b with the factors of TRs [t0, t0 + 2^cs) (clamped to T - 1 past the
+// end) for the wave's G groups; called by all 64 lanes.
+template <bool PHASE, bool HREAL>
+__device__ __forceinline__ void fill_table(TabBuf &b, const TrCoef *__restrict__ cf, int T, int t0,
+                                           int cs, int G, int lead, float T2, float B0, int lane)
+{
+    const int chunk = 1 << cs;
+    const int ne = G << cs;
+    for (int e0 = 0; e0 < ne; e0 += 64) {
+        const int e = e0 + lane;
+        const int g = min(e >> cs, G - 1);
+        const int ld = __shfl(lead, g);
+        const float t2 = __shfl(T2, ld), b0 = __shfl(B0, ld);
+        const int t = min(t0 + (e & (chunk - 1)), T - 1);
+        const float4 f = group_factors<PHASE, HREAL>(cf[t], decay_k(t2), (double)b0 * 1e-3);
+        if (e < ne) b.grp[e] = f;
+    }
+    if (lane < chunk) b.trc[lane] = slice_factors(cf[min(t0 + lane, T - 1)]);
+}
+
 template <int M, bool MULTI, bool PHASE, bool HREAL, bool TRUNC>
 __device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *__restrict__ coef,
                                                const float *__restrict__ params,
@@ -412,25 +516,56 @@ __device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *_
         w.T = a.T;
         w.nvalid = first < a.natoms ? (int)((a.natoms - first) < 64 ? (a.natoms - first) : 64) : 0;
         w.lane = threadIdx.x & 63;
-        int t0 = 0;
+        __shared__ TabBuf tab_lds[kBlock / 64][2];
+        TabBuf *tb = tab_lds[threadIdx.x >> 6];
+        int my_g, lead;
+        const int G = wave_groups(T2, B0, w.lane, &my_g, &lead);
+        // chunk = 2^cs TRs: 64 for G <= 4, halved per doubling of G
+        const int cs = min(6, 8 - (32 - __builtin_clz((unsigned)(2 * G - 1)) - 1));
+        const int chunk = 1 << cs;
+        fill_table<PHASE, HREAL>(tb[0], cf, a.T, 0, cs, G, lead, T2, B0, w.lane);
+        // one flat TR loop (a nested chunk loop makes the register allocator
+        // move the state between the loops); at the first TR of each chunk
+        // the next chunk's table goes into the other buffer
+        int t0 = 0;    // first TR of the staged segment
+        int kk = 0;    // TR index within the chunk
+        int cb = 0;    // buffer of the current chunk
         for (; n + kUnroll <= a.T; n += kUnroll) {
-            const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
-            const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
+            if (kk == 0) {
+                wave_lds_sync();
+                if (n + chunk < a.T)
+                    fill_table<PHASE, HREAL>(tb[cb ^ 1], cf, a.T, n + chunk, cs, G, lead, T2, B0,
+                                             w.lane);
+                wave_lds_sync();
+            }
+            const TabBuf &tab = tb[cb];
+            const float4 *gp = &tab.grp[my_g << cs];
+            const float2 v0 = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, gp[kk], tab.trc[kk], x, a);
+            const float2 v1 = fused_core<M, 1, MULTI, HREAL, TRUNC>(s, gp[kk + 1], tab.trc[kk + 1],
+                                                                    x, a);
             fm_copy_down<M, 2>(s);
             stage_put(w, n - t0, v0);
             stage_put(w, n + 1 - t0, v1);
             nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
             nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
+            kk += kUnroll;
+            if (kk == chunk) {
+                kk = 0;
+                cb ^= 1;
+            }
             if (n + 2 - t0 == kSeg) {
                 stage_flush(w, t0, kSeg, even_t);
                 t0 += kSeg;
             }
         }
-        for (; n < a.T; n++) {
-            const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
+        if (n < a.T) {   // odd T: the last TR (its chunk's table is filled)
+            wave_lds_sync();
+            const float2 v = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, tb[cb].grp[(my_g << cs) + kk],
+                                                                   tb[cb].trc[kk], x, a);
             fm_copy_down<M, 1>(s);
             stage_put(w, n - t0, v);
             nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            n++;
         }
         if (a.T > t0) stage_flush(w, t0, a.T - t0, even_t);
         if (a.normalize) {

@@ -206,6 +206,26 @@ def test_lane_forms_vs_oracle(gpu, echo, w, T, lanes, normalize):
         assert rel_err(unit, ref) < TOL
 
 
+@pytest.mark.parametrize("groups,T", [(1, 1000), (3, 131), (13, 57), (64, 37)])
+def test_one_lane_form_group_counts(gpu, groups, T):
+    """The one-lane form forms each TR's T2/B0 factors once per wave group of
+    equal (T2, B0), a chunk of TRs at a time (64 TRs for up to 4 groups, 16
+    for 13, 4 for 64): waves whose 64 atoms hold 1, 3, 13 and 64 distinct
+    (T2, B0) pairs (random T1 in each) against the FP64 oracle, odd T included."""
+    rng = np.random.default_rng(groups)
+    A = 640
+    key = (np.arange(A) % 64) * groups // 64 + 1000 * (np.arange(A) // 64)
+    _, inv = np.unique(key, return_inverse=True)
+    t2 = rng.uniform(20, 300, inv.max() + 1)[inv]
+    b0 = rng.uniform(-40, 40, inv.max() + 1)[inv]
+    params = np.stack([rng.uniform(200, 3000, A), t2, b0, np.ones(A)], 1).astype(np.float32)
+    s = synth.sequence(T, seed=groups)
+    raw = synth.csv_columns(s, "epg_ssfp")
+    got = gen(gpu, "epg_ssfp", raw, params, A, 30, lanes=1)
+    ref = oracle.epg("epg_ssfp", oracle.seq_prepare("epg_ssfp", raw), params, A, 30)
+    assert rel_err(got, ref) < TOL
+
+
 def test_lanes_per_atom_rejects_bad_values(gpu):
     raw = synth.csv_columns(synth.sequence(20, seed=0), "epg_ssfp")
     params, A, natoms = sm.build_params(*LANE_GRID)
