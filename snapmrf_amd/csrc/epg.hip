@@ -305,24 +305,24 @@ __device__ __forceinline__ void fm_copy_down(FusedState<M> &s)
     for (int k = 0; k < M; k++) s.fm[k] = s.fm[k + SH];
 }
 
-// Output staging for one-lane-per-atom waves.  Each lane's samples go to its
-// row of a per-wave LDS tile; every kSeg TRs the wave writes its 64 atom
-// rows as whole 128-byte segments (8 lanes x 16 bytes per row), so every
-// HBM line is written once and completely.
+// Output staging.  Each atom's samples go to its row of a per-wave LDS tile
+// (64 / L rows for L lanes per atom; the group's lane 0 holds the sample);
+// every kSeg TRs the wave writes its rows as whole 128-byte segments (8 lanes
+// x 16 bytes per row), so every HBM line is written once and completely.
 constexpr int kSeg = 16;
 constexpr int kSegStride = 2 * kSeg + 2;   // floats per LDS row (padding spreads banks)
 
 struct WaveStage {
-    float *lds;        // this wave's [64][kSegStride] floats
+    float *lds;        // this wave's [rows][kSegStride] floats
     float2 *rows;      // first atom row of this wave
     int T;
     int nvalid;        // atom rows of this wave that exist
     int lane;
 };
 
-__device__ __forceinline__ void stage_put(const WaveStage &w, int slot, float2 v)
+__device__ __forceinline__ void stage_put(const WaveStage &w, int row, int slot, float2 v)
 {
-    *reinterpret_cast<float2 *>(&w.lds[w.lane * kSegStride + 2 * slot]) = v;
+    *reinterpret_cast<float2 *>(&w.lds[row * kSegStride + 2 * slot]) = v;
 }
 
 // Writes TRs [t0, t0 + count) of the wave's rows.
@@ -363,14 +363,13 @@ __device__ __forceinline__ float4 scale_v(float4 v, float f)
 __device__ __forceinline__ float2 scale_v(float2 v, float f) { return make_float2(v.x * f, v.y * f); }
 
 // Scales the wave's nvalid contiguous rows by the factors in w.lds[row];
-// VEC elements, per_row of them per row.
-template <typename VEC>
+// VEC elements, per_row of them per row, kDepth loads in flight per lane.
+template <typename VEC, int kDepth>
 __device__ __forceinline__ void normalize_rows(const WaveStage &w, int per_row)
 {
     VEC *base = reinterpret_cast<VEC *>(w.rows);
     const int total = w.nvalid * per_row;
     const float inv = 1.0f / (float)per_row;
-    constexpr int kDepth = 32;   // loads in flight per lane
     for (int i0 = w.lane; i0 < total; i0 += 64 * kDepth) {
         VEC v[kDepth];
 #pragma unroll
@@ -391,19 +390,19 @@ __device__ __forceinline__ void normalize_rows(const WaveStage &w, int per_row)
     }
 }
 
-// Per-TR factor tables of one-lane-per-atom waves.  The per-TR factors that
-// do not depend on T1 (two exp2, two sincos with their FP64 range reduction,
-// the complex products of group_factors) are the same for every atom of a
-// wave with the same (T2, B0): in the reference's atom order (T1 fastest) a
-// 64-atom wave holds one or two such groups.  The wave finds its G groups,
-// and its lanes form the group factors of a chunk of TRs together (G x chunk
-// entries, chunk = 64 TRs for G <= 4, fewer for more groups, so a chunk never
-// has more than 256 entries), together with the slice factors of the chunk's
-// TRs; the TR loop then reads each TR's factors from LDS with two broadcast
-// loads instead of forming them per lane.  Two buffers: the next chunk is
-// filled while the current one is read.
-constexpr int kTabGroupEntries = 256;
-constexpr int kTabChunkMax = 64;
+// Per-TR factor tables.  The per-TR factors that do not depend on T1 (two
+// exp2, two sincos with their FP64 range reduction, the complex products of
+// group_factors) are the same for every atom of a wave with the same
+// (T2, B0): in the reference's atom order (T1 fastest) a wave holds one or
+// two such groups.  The wave finds its G groups, and its lanes form the group
+// factors of a chunk of TRs together (G x chunk entries, chunk = 32 TRs for
+// G <= 4, fewer for more groups, so a chunk never has more than 128
+// entries), together with the slice factors of the chunk's TRs; the TR loop
+// then reads each TR's factors from LDS with two broadcast loads instead of
+// forming them per lane.  Two buffers: the next chunk is filled while the
+// current one is read.
+constexpr int kTabGroupEntries = 128;
+constexpr int kTabChunkMax = 32;
 
 struct TabBuf {
     float4 grp[kTabGroupEntries];   // [group][TR of the chunk] group factors
@@ -502,118 +501,101 @@ __device__ __forceinline__ void epg_fused_body(const EpgArgs &a, const TrCoef *_
         s.fm[0] = cmul(z0, s.fm[0]);
     }
 
-    float2 *__restrict__ out = atoms + ((size_t)slice * a.natoms + pa) * (size_t)a.T;
     const bool store = valid && x.j == 0;
     const bool even_t = (a.T & 1) == 0;
     double nrm = 0.0;
     int n = 0;
-    if constexpr (!MULTI) {
-        __shared__ __attribute__((aligned(16))) float stage_lds[kBlock / 64][64 * kSegStride];
-        const size_t first = (size_t)blockIdx.x * kBlock + (threadIdx.x & ~63u);
-        WaveStage w;
-        w.lds = stage_lds[threadIdx.x >> 6];
-        w.rows = atoms + ((size_t)slice * a.natoms + first) * (size_t)a.T;
-        w.T = a.T;
-        w.nvalid = first < a.natoms ? (int)((a.natoms - first) < 64 ? (a.natoms - first) : 64) : 0;
-        w.lane = threadIdx.x & 63;
-        __shared__ TabBuf tab_lds[kBlock / 64][2];
-        TabBuf *tb = tab_lds[threadIdx.x >> 6];
-        int my_g, lead;
-        const int G = wave_groups(T2, B0, w.lane, &my_g, &lead);
-        // chunk = 2^cs TRs: 64 for G <= 4, halved per doubling of G
-        const int cs = min(6, 8 - (32 - __builtin_clz((unsigned)(2 * G - 1)) - 1));
-        const int chunk = 1 << cs;
-        fill_table<PHASE, HREAL>(tb[0], cf, a.T, 0, cs, G, lead, T2, B0, w.lane);
-        // one flat TR loop (a nested chunk loop makes the register allocator
-        // move the state between the loops); at the first TR of each chunk
-        // the next chunk's table goes into the other buffer
-        int t0 = 0;    // first TR of the staged segment
-        int kk = 0;    // TR index within the chunk
-        int cb = 0;    // buffer of the current chunk
-        for (; n + kUnroll <= a.T; n += kUnroll) {
-            if (kk == 0) {
-                wave_lds_sync();
-                if (n + chunk < a.T)
-                    fill_table<PHASE, HREAL>(tb[cb ^ 1], cf, a.T, n + chunk, cs, G, lead, T2, B0,
-                                             w.lane);
-                wave_lds_sync();
-            }
-            const TabBuf &tab = tb[cb];
-            const float4 *gp = &tab.grp[my_g << cs];
-            const float2 v0 = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, gp[kk], tab.trc[kk], x, a);
-            const float2 v1 = fused_core<M, 1, MULTI, HREAL, TRUNC>(s, gp[kk + 1], tab.trc[kk + 1],
-                                                                    x, a);
-            fm_copy_down<M, 2>(s);
-            stage_put(w, n - t0, v0);
-            stage_put(w, n + 1 - t0, v1);
-            nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
-            nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
-            kk += kUnroll;
-            if (kk == chunk) {
-                kk = 0;
-                cb ^= 1;
-            }
-            if (n + 2 - t0 == kSeg) {
-                stage_flush(w, t0, kSeg, even_t);
-                t0 += kSeg;
-            }
-        }
-        if (n < a.T) {   // odd T: the last TR (its chunk's table is filled)
+    // the wave's atoms are contiguous rows: 64 / L of them
+    constexpr int kRows = MULTI ? 32 : 64;
+    const int lshift = MULTI ? a.lshift : 0;
+    __shared__ __attribute__((aligned(16))) float stage_lds[kBlock / 64][kRows * kSegStride];
+    const size_t first = ((size_t)blockIdx.x * kBlock + (threadIdx.x & ~63u)) >> lshift;
+    const int rows = 64 >> lshift;
+    WaveStage w;
+    w.lds = stage_lds[threadIdx.x >> 6];
+    w.rows = atoms + ((size_t)slice * a.natoms + first) * (size_t)a.T;
+    w.T = a.T;
+    w.nvalid = first < a.natoms ? (int)((a.natoms - first) < (size_t)rows ? (a.natoms - first) : rows)
+                                : 0;
+    w.lane = threadIdx.x & 63;
+    const int row = w.lane >> lshift;
+    __shared__ TabBuf tab_lds[kBlock / 64][2];
+    TabBuf *tb = tab_lds[threadIdx.x >> 6];
+    int my_g, lead;
+    const int G = wave_groups(T2, B0, w.lane, &my_g, &lead);
+    // chunk = 2^cs TRs: 32 for G <= 4, halved per doubling of G (2 for 64)
+    const int cs = min(5, 7 - (32 - __builtin_clz((unsigned)(2 * G - 1)) - 1));
+    const int chunk = 1 << cs;
+    fill_table<PHASE, HREAL>(tb[0], cf, a.T, 0, cs, G, lead, T2, B0, w.lane);
+    // one flat TR loop (a nested chunk loop makes the register allocator move
+    // the state between the loops); at the first TR of each chunk the next
+    // chunk's table goes into the other buffer
+    int t0 = 0;    // first TR of the staged segment
+    int kk = 0;    // TR index within the chunk
+    int cb = 0;    // buffer of the current chunk
+    for (; n + kUnroll <= a.T; n += kUnroll) {
+        if (kk == 0) {
             wave_lds_sync();
-            const float2 v = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, tb[cb].grp[(my_g << cs) + kk],
-                                                                   tb[cb].trc[kk], x, a);
-            fm_copy_down<M, 1>(s);
-            stage_put(w, n - t0, v);
-            nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-            n++;
+            if (n + chunk < a.T)
+                fill_table<PHASE, HREAL>(tb[cb ^ 1], cf, a.T, n + chunk, cs, G, lead, T2, B0,
+                                         w.lane);
+            wave_lds_sync();
         }
-        if (a.T > t0) stage_flush(w, t0, a.T - t0, even_t);
-        if (a.normalize) {
-            // normalize (src/functions.cu:670-685) on the wave's own rows,
-            // which are contiguous: per-row factors go to LDS, then the wave
-            // streams the block with 32 independent loads in flight per lane.
-            // (the wave re-reads its own stores through its CU's caches:
-            // workgroup scope suffices and avoids an L2 write-back)
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            w.lds[w.lane] = nrm > 0.0 ? (float)(1.0 / sqrt(nrm)) : 1.0f;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (even_t) normalize_rows<float4>(w, a.T / 2);
-            else normalize_rows<float2>(w, a.T);
+        const TabBuf &tab = tb[cb];
+        const float4 *gp = &tab.grp[my_g << cs];
+        const float2 v0 = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, gp[kk], tab.trc[kk], x, a);
+        const float2 v1 = fused_core<M, 1, MULTI, HREAL, TRUNC>(s, gp[kk + 1], tab.trc[kk + 1], x,
+                                                                a);
+        fm_copy_down<M, 2>(s);
+        if (x.j == 0) {
+            stage_put(w, row, n - t0, v0);
+            stage_put(w, row, n + 1 - t0, v1);
         }
-    } else {
-        for (; n + kUnroll <= a.T; n += kUnroll) {
-            const float2 v0 = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
-            const float2 v1 = fused_step<M, 1, MULTI, PHASE, HREAL, TRUNC>(s, cf[n + 1], x, a);
-            fm_copy_down<M, 2>(s);
-            if (store) {
-                if (even_t) {
-                    *reinterpret_cast<float4 *>(out + n) = make_float4(v0.x, v0.y, v1.x, v1.y);
-                } else {
-                    out[n] = v0;
-                    out[n + 1] = v1;
-                }
-                nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
-                nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
-            }
+        nrm += (double)v0.x * (double)v0.x + (double)v0.y * (double)v0.y;
+        nrm += (double)v1.x * (double)v1.x + (double)v1.y * (double)v1.y;
+        kk += kUnroll;
+        if (kk == chunk) {
+            kk = 0;
+            cb ^= 1;
         }
-        for (; n < a.T; n++) {
-            const float2 v = fused_step<M, 0, MULTI, PHASE, HREAL, TRUNC>(s, cf[n], x, a);
-            fm_copy_down<M, 1>(s);
-            if (store) {
-                out[n] = v;
-                nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-            }
+        if (n + 2 - t0 == kSeg) {
+            stage_flush(w, t0, kSeg, even_t);
+            t0 += kSeg;
         }
+    }
+    if (n < a.T) {   // odd T: the last TR (its chunk's table is filled)
+        wave_lds_sync();
+        const float2 v = fused_core<M, 0, MULTI, HREAL, TRUNC>(s, tb[cb].grp[(my_g << cs) + kk],
+                                                               tb[cb].trc[kk], x, a);
+        fm_copy_down<M, 1>(s);
+        if (x.j == 0) stage_put(w, row, n - t0, v);
+        nrm += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        n++;
+    }
+    if (a.T > t0) stage_flush(w, t0, a.T - t0, even_t);
+    if (a.normalize) {
+        // normalize (src/functions.cu:670-685) on the wave's own rows, which
+        // are contiguous: per-row factors go to LDS, then the wave streams the
+        // block with 32 independent loads in flight per lane.  (the wave
+        // re-reads its own stores through its CU's caches: workgroup scope
+        // suffices and avoids an L2 write-back)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (x.j == 0) w.lds[row] = nrm > 0.0 ? (float)(1.0 / sqrt(nrm)) : 1.0f;
+        wave_lds_sync();
+        // (16 loads in flight for the lane-group forms, whose 3 waves per
+        // SIMD leave 168 registers)
+        constexpr int kDepth = MULTI ? 16 : 32;
+        if (even_t) normalize_rows<float4, kDepth>(w, a.T / 2);
+        else normalize_rows<float2, kDepth>(w, a.T);
     }
     if (store && a.norm2) a.norm2[(size_t)slice * a.natoms + atom] = nrm;
 }
 
-// Two occupancy classes: without truncation the state fits 256 VGPRs
-// (2 waves per SIMD); the truncating variants get the full register file.
+// Occupancy classes: without truncation the state fits 256 VGPRs (2 waves
+// per SIMD), or 168 (3 waves) for up to 16 states per lane; the truncating
+// variants get the full register file.
 template <int M, bool MULTI, bool PHASE, bool HREAL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(M <= 16 ? 3 : 2)))
 void epg_fused_kernel(EpgArgs a, const TrCoef *__restrict__ coef,
                       const float *__restrict__ params, float2 *__restrict__ atoms)
 {
@@ -913,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void normalize_kernel(float2 *atoms, const 
 // ------------------------------------------------------------------------
 
 // Compiled state counts per lane.
-constexpr int kMs[] = {4, 8, 12, 16, 20, 24, 28, 30, 32};
+constexpr int kMs[] = {4, 8, 10, 12, 15, 16, 20, 24, 28, 30, 32};
 
 template <int M>
 int launch_m(const EpgLaunch &l, const EpgArgs &args, dim3 grid, hipStream_t s)
@@ -1131,7 +1113,7 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
     args.init_alpha = (float)((double)a_deg * M_PI / 180.0);
     args.init_phi = (float)((double)p_deg * M_PI / 180.0);
     args.TI = TI;
-    args.normalize = (l.normalize && fused && L == 1) ? 1 : 0;
+    args.normalize = (l.normalize && fused) ? 1 : 0;
 
     const size_t lanes = l.natoms * (size_t)L;
     const size_t blocks = (lanes + kBlock - 1) / kBlock;
@@ -1141,7 +1123,9 @@ int launch_epg(const EpgLaunch &l, hipStream_t s)
     switch (M) {
     case 4: rc = launch_m<4>(l, args, grid, s); break;
     case 8: rc = launch_m<8>(l, args, grid, s); break;
+    case 10: rc = launch_m<10>(l, args, grid, s); break;
     case 12: rc = launch_m<12>(l, args, grid, s); break;
+    case 15: rc = launch_m<15>(l, args, grid, s); break;
     case 16: rc = launch_m<16>(l, args, grid, s); break;
     case 20: rc = launch_m<20>(l, args, grid, s); break;
     case 24: rc = launch_m<24>(l, args, grid, s); break;

@@ -77,7 +77,7 @@ def test_ssfp_b0_b1_grid_vs_oracle(gpu, w):
     """B1-sliced SSFP with off-resonance on 1,530 atoms.  A launch this small
     is spread over more lanes per atom (until it has 2^17 lanes, at least 8
     states per lane): w = 30 runs 4 lanes of 8 states, 60 runs 8 x 8, 101
-    and 200 run 16 x 8 and 16 x 16 (padded, truncated at w - 1).  The
+    and 200 run 16 x 8 and 16 x 15 (padded, truncated at w - 1).  The
     one-lane form is tested by test_lane_forms_vs_oracle and at full size by
     test_full_grid_one_lane_form_vs_oracle."""
     T = 400 if w > 60 else 1000
@@ -174,8 +174,10 @@ LANE_GRID = ("100:300:2000+2300:900:5000", "20:40:300+400:800:2000", "-40:20:40"
     ("epg_ssfp", 7, 30, 1, -1),      # T below two 16-TR output segments
     ("epg_ssfp", 30, 5, 1, -1),      # T shorter than one segment
     ("epg_fisp", 20, 1000, 1, 0),    # constant-phase FISP branch
-    ("epg_ssfp", 30, 500, 2, 1),     # lane pairs of 16 (truncated), separate normalisation
+    ("epg_ssfp", 30, 500, 2, 1),     # lane pairs of 15 states, in-kernel normalisation
     ("epg_ssfp", 30, 400, 16, 1),    # 16 lanes of 4 states (64, truncated at w - 1)
+    ("epg_ssfp", 20, 999, 4, 1),     # 4 lanes of 8 states (truncated), odd T, in-kernel normalisation
+    ("epg_ssfp", 32, 1000, 2, 0),    # lane pairs of 16 states, norms returned
 ])
 def test_lane_forms_vs_oracle(gpu, echo, w, T, lanes, normalize):
     """Each lane form of the fused kernel, forced through
