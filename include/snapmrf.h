@@ -289,11 +289,12 @@ int mrf_epg_jacobian(void *stream, const float *h_mrf, size_t nreps, size_t NREP
 /* Reverse mode for L = sum_atoms Re sum_n conj(cot_n) s_n with d_cot
  * [n][nreps]: d_grad_theta [n][4] = dL/dtheta (optional) and d_grad_seq
  * [2][nreps] += (dL/dFA_raw, dL/dTR_raw) summed over atoms (optional,
- * accumulated: zero it first).  States are checkpointed every 50 TRs and
- * recomputed per segment.  Scratch: about 4 * (ceil(nreps / 50) + 50) * 6 *
- * nstates bytes per atom (5x that for mrf_crb_loss_grad), allocated per call
- * on the stream; calls needing more than 4 GiB (or MRF_DIFF_SCRATCH_BYTES when
- * set in the environment) run the atoms in equal batches. */
+ * accumulated: zero it first).  States are checkpointed every 4 TRs and
+ * recomputed per segment.  Scratch: 4 * (96 L ceil(nreps / 4) + 2 nreps)
+ * bytes per atom, L = 8 for nstates <= 32 and 16 above (about 200 KB at
+ * nreps = 1000, nstates = 30), allocated per call on the stream; calls
+ * needing more than 16 GiB (or MRF_DIFF_SCRATCH_BYTES when set in the
+ * environment) run the atoms in equal batches. */
 int mrf_epg_adjoint(void *stream, const float *h_mrf, size_t nreps, size_t NREPS,
                     const float *d_theta, size_t n, int nstates, const mrf_complex *d_cot,
                     float *d_grad_theta, float *d_grad_seq);

@@ -649,10 +649,10 @@ void build_gn_coef(const float *h_mrf, size_t nreps, size_t NREPS, std::vector<T
 }
 
 constexpr float kFaUnit = (float)(60.0 * M_PI / 180.0);
-constexpr size_t kDiffScratchBudget = (size_t)4 << 30;   // bytes per launch batch
+constexpr size_t kDiffScratchBudget = (size_t)16 << 30;   // bytes per launch batch
 
 // scratch bytes per launch batch: MRF_DIFF_SCRATCH_BYTES when set (read at
-// every call), else 4 GiB
+// every call), else 16 GiB
 size_t diff_budget()
 {
     if (const char *e = std::getenv("MRF_DIFF_SCRATCH_BYTES")) {

@@ -1,7 +1,7 @@
 // grad.hip -- differentiation form of the EPG-SSFP program: Jacobians with
 // respect to (T1, T2, B0, B1), reverse-mode adjoints with respect to those
 // and to the per-timepoint flip angles and repetition times (checkpointed
-// recomputation every kCkpt TRs), per-voxel Gauss-Newton refinement, and the
+// recomputation), per-voxel Gauss-Newton refinement, and the
 // gradient of a Cramer-Rao-bound sequence-design loss.
 //
 // Model: the reference's epg_ssfp program (src/functions.cu:936-1019 with
@@ -13,7 +13,9 @@
 // Layout: one group of SUB lanes per atom (SUB = 32 for w <= 32, two atoms
 // per wave; else 64), lane k holds EPG state k (F+_k, F-_k, Z_k) and its ND
 // tangents, so an atom's whole state sits in 6 (1 + ND) registers per lane;
-// the dephasing shift is a cross-lane move inside the group.  Per TR n:
+// the dephasing shift is a cross-lane move inside the group (Jacobian,
+// Gauss-Newton, CRB).  The adjoint puts an atom on 8 or 16 lanes with four
+// states per lane (adj_kernel).  Per TR n:
 //   RF(alpha_n B1, phi_n) -> relax(TE_n) -> phase(TE_n) -> s_n = F+_0 (lane 0)
 //   -> phase(TR_n - TE_n) -> dephase -> relax(TR_n - TE_n)
 // TR n runs in the frame of its RF phase, x~ = P_n^-1 x with
@@ -38,16 +40,7 @@ namespace mrf {
 
 namespace {
 
-constexpr int kCkpt = 50;                 // TRs between checkpoints (ND = 4)
-// The adjoint (ND = 0) checkpoints every kSegLds TRs and keeps a segment's
-// TR-start states in LDS (kSegLds x 3 float2 per lane) rather than in a
-// global segment buffer: one state write and read per TR in LDS instead of
-// 24 bytes per lane and TR to and from HBM, for 50 / kSegLds times the
-// checkpoint traffic.  Measured on 57,600 atoms (T = 1000, w = 30): 2, 3, 5,
-// 8, 10, 16 TRs per segment took 22.4, 21.8, 19.1, 22.1, 21.6, 38.0 ms
-// (5 is the longest segment at which three work-groups fit a CU's LDS; the
-// 50-TR global segment buffer took 22.2 ms).
-constexpr int kSegLds = 5;
+constexpr int kCkpt = 50;                 // TRs between checkpoints (CRB)
 constexpr int kWaves = 4;                 // atoms per work-group
 constexpr float kCB0 = (float)(2.0 * M_PI / 1000.0);
 constexpr float kTI = 40.0f;              // inversion delay, functions.cu:1242
@@ -103,21 +96,6 @@ template <int SUB>
 __device__ __forceinline__ float2 bcast0(float2 v)
 {
     return {bcast0<SUB>(v.x), bcast0<SUB>(v.y)};
-}
-// Entry i (uniform) of a per-group table held across the group's lanes: lane
-// k holds entry k in c0 and entry 32 + k in c1 (SUB = 32; SUB = 64 uses c0)
-template <int SUB>
-__device__ __forceinline__ float2 group_pick(float2 c0, float2 c1, int i)
-{
-    const float2 src = (SUB == 32 && i >= 32) ? c1 : c0;
-    const int li = SUB == 32 ? (i & 31) : i;
-    auto rl = [](float v, int lane) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-    };
-    const float2 a = {rl(src.x, li), rl(src.y, li)};
-    if (SUB == 64) return a;
-    const float2 b = {rl(src.x, li + 32), rl(src.y, li + 32)};
-    return (threadIdx.x & 32) ? b : a;
 }
 
 // v + the value of the lane given by a DPP pattern
@@ -752,44 +730,18 @@ __device__ bool inv4(const double *A, double *X)
     return true;
 }
 
-// One atom's sweeps (the lane group of SUB lanes); gs_lds is the work-group's
-// grad_seq partial sums when A.seq_lds.
-// checkpoint interval of a reverse sweep
-template <int ND>
-constexpr int ckpt_of() { return ND == 0 ? kSegLds : kCkpt; }
-
-// a lane's TR-start state in the LDS segment buffer (ND = 0): slot [tr][3]
-// of the wave's 64 lanes, float2 each
-__device__ __forceinline__ void seg_put(float2 *seg, int tr, const Aug<0> &y)
-{
-    const int l = threadIdx.x & 63;
-    seg[(tr * 3 + 0) * 64 + l] = y.fp[0];
-    seg[(tr * 3 + 1) * 64 + l] = y.fm[0];
-    seg[(tr * 3 + 2) * 64 + l] = y.z[0];
-}
-
-__device__ __forceinline__ void seg_get(const float2 *seg, int tr, int w, int k, Aug<0> &y)
-{
-    const int l = threadIdx.x & 63;
-    if (k < w) {
-        y.fp[0] = seg[(tr * 3 + 0) * 64 + l];
-        y.fm[0] = seg[(tr * 3 + 1) * 64 + l];
-        y.z[0] = seg[(tr * 3 + 2) * 64 + l];
-    } else {
-        y.fp[0] = y.fm[0] = y.z[0] = {0.f, 0.f};
-    }
-}
-
+// One atom's sweeps of the CRB gradient (the lane group of SUB lanes, lane k
+// = state k); gs_lds is the work-group's grad_seq partial sums when A.seq_lds.
 template <int ND, int SUB>
-__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds, TrA *coef_lds,
-                                         float2 *seg_lds)
+__device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_lds, TrA *coef_lds)
 {
-    constexpr int CKPT = ckpt_of<ND>();
+    static_assert(ND == 4, "the adjoint (ND = 0) is adj_kernel");
+    constexpr int CKPT = kCkpt;
     const int k = threadIdx.x & (SUB - 1);
     const int w = A.w, T = A.T;
     const int C = 6 * (1 + ND) * w;
-    float *ck = A.scratch + a * (size_t)(A.nck + (ND == 0 ? 0 : kCkpt)) * C;
-    float *seg = ck + (size_t)A.nck * C;      // ND = 4: global segment buffer
+    float *ck = A.scratch + a * (size_t)(A.nck + kCkpt) * C;
+    float *seg = ck + (size_t)A.nck * C;      // the segment's TR-start states
     const Atom p = load_atom(A.theta, a);
     CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
 
@@ -804,54 +756,48 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     for (int t = 0; t < T; t++) {
         if (t % CKPT == 0) st_store(ck + (size_t)(t / CKPT) * C, w, k, y);
         const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
-        if constexpr (ND > 0) {
-            Aug<ND> r;
-            f_tr<ND, SUB>(y, c, p, k, w, &r);
-            if (k == 0) {
-                int i = 0;
+        Aug<ND> r;
+        f_tr<ND, SUB>(y, c, p, k, w, &r);
+        if (k == 0) {
+            int i = 0;
 #pragma unroll
-                for (int d = 0; d < 4; d++)
+            for (int d = 0; d < 4; d++)
 #pragma unroll
-                    for (int e = d; e < 4; e++) F[i++] += re_dot(r.fp[1 + d], r.fp[1 + e]);
-            }
-        } else {
-            f_tr<ND, SUB>(y, c, p, k, w, nullptr);
+                for (int e = d; e < 4; e++) F[i++] += re_dot(r.fp[1 + d], r.fp[1 + e]);
         }
     }
-    // loss and dL/dF (ND = 4), on lane 0 in FP64, broadcast as G
+    // loss and dL/dF, on lane 0 in FP64, broadcast as G
     float G[10];
 #pragma unroll
     for (int i = 0; i < 10; i++) G[i] = 0.f;
-    if constexpr (ND > 0) {
-        if (k == 0) {
-            double Fd[16], Fi[16];
-            int i = 0;
-            for (int d = 0; d < 4; d++)
-                for (int e = d; e < 4; e++, i++) Fd[4 * d + e] = Fd[4 * e + d] = F[i];
-            double loss = 0.0;
-            if (inv4(Fd, Fi)) {
-                double Wd[4];
-                for (int d = 0; d < 4; d++) {
-                    const double sc = d == 0 ? p.T1 : (d == 1 ? p.T2 : 1.0);
-                    Wd[d] = A.wts[d] / (sc * sc);
-                    loss += Wd[d] * Fi[5 * d];
-                }
-                // dL/dF = -F^-1 W F^-1
-                i = 0;
-                for (int d = 0; d < 4; d++)
-                    for (int e = d; e < 4; e++, i++) {
-                        double g = 0.0;
-                        for (int m = 0; m < 4; m++) g -= Fi[4 * d + m] * Wd[m] * Fi[4 * m + e];
-                        G[i] = (float)g;
-                    }
-            } else {
-                loss = __builtin_nan("");
+    if (k == 0) {
+        double Fd[16], Fi[16];
+        int i = 0;
+        for (int d = 0; d < 4; d++)
+            for (int e = d; e < 4; e++, i++) Fd[4 * d + e] = Fd[4 * e + d] = F[i];
+        double loss = 0.0;
+        if (inv4(Fd, Fi)) {
+            double Wd[4];
+            for (int d = 0; d < 4; d++) {
+                const double sc = d == 0 ? p.T1 : (d == 1 ? p.T2 : 1.0);
+                Wd[d] = A.wts[d] / (sc * sc);
+                loss += Wd[d] * Fi[5 * d];
             }
-            A.loss[a] = loss;
+            // dL/dF = -F^-1 W F^-1
+            i = 0;
+            for (int d = 0; d < 4; d++)
+                for (int e = d; e < 4; e++, i++) {
+                    double g = 0.0;
+                    for (int m = 0; m < 4; m++) g -= Fi[4 * d + m] * Wd[m] * Fi[4 * m + e];
+                    G[i] = (float)g;
+                }
+        } else {
+            loss = __builtin_nan("");
         }
-#pragma unroll
-        for (int i = 0; i < 10; i++) G[i] = bcast0<SUB>(G[i]);
+        A.loss[a] = loss;
     }
+#pragma unroll
+    for (int i = 0; i < 10; i++) G[i] = bcast0<SUB>(G[i]);
 
     // reverse sweep, segment by segment
     Aug<ND> l;
@@ -859,45 +805,26 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     for (int j = 0; j <= ND; j++) l.fp[j] = l.fm[j] = l.z[j] = {0.f, 0.f};
     float gth[4] = {0.f, 0.f, 0.f, 0.f};
     const float beta = kCB0 * p.B0;
-    static_assert(CKPT <= 64, "a segment's cotangents live in two registers per lane");
-    // ND = 0: the checkpoint and cotangents of a segment are loaded while
-    // the segment after it (in time) is reversed
+    // the checkpoint of a segment is loaded while the segment after it (in
+    // time) is reversed
     Aug<ND> cknext;
-    float2 cvn0 = {0.f, 0.f}, cvn1 = {0.f, 0.f};
-    auto seg_inputs = [&](int sgi, Aug<ND> &ckp, float2 &c0, float2 &c1) {
-        const int t0 = sgi * CKPT, t1 = min(T, t0 + CKPT);
-        st_load(ck + (size_t)sgi * C, w, k, ckp);
-        c0 = c1 = {0.f, 0.f};
-        if constexpr (ND == 0) {
-            const float2 *cr = A.cot + a * T;
-            if (t0 + k < t1) c0 = cr[t0 + k];
-            if (SUB == 32 && t0 + 32 + k < t1) c1 = cr[t0 + 32 + k];
-        }
-    };
-    seg_inputs(A.nck - 1, cknext, cvn0, cvn1);
+    st_load(ck + (size_t)(A.nck - 1) * C, w, k, cknext);
     for (int sgi = A.nck - 1; sgi >= 0; sgi--) {
         const int t0 = sgi * CKPT;
         const int t1 = min(T, t0 + CKPT);
-        // the segment's cotangents, read back per TR by readlane
-        const float2 cv0 = cvn0, cv1 = cvn1;
         y = cknext;
-        if (sgi > 0) seg_inputs(sgi - 1, cknext, cvn0, cvn1);
+        if (sgi > 0) st_load(ck + (size_t)(sgi - 1) * C, w, k, cknext);
         for (int t = t0; t < t1; t++) {
-            if constexpr (ND == 0) seg_put(seg_lds, t - t0, y);
-            else st_store(seg + (size_t)(t - t0) * C, w, k, y);
+            st_store(seg + (size_t)(t - t0) * C, w, k, y);
             f_tr<ND, SUB>(y, coef<SUB>(cc, A.seq, T, p, t, k), p, k, w, nullptr);
         }
         // the TR-start state of the next (earlier) TR is loaded while this
         // TR is reversed
         Aug<ND> ynext;
-        if constexpr (ND == 0) seg_get(seg_lds, t1 - 1 - t0, w, k, ynext);
-        else st_load(seg + (size_t)(t1 - 1 - t0) * C, w, k, ynext);
+        st_load(seg + (size_t)(t1 - 1 - t0) * C, w, k, ynext);
         for (int t = t1 - 1; t >= t0; t--) {
             const Aug<ND> yn = ynext;
-            if (t > t0) {
-                if constexpr (ND == 0) seg_get(seg_lds, t - 1 - t0, w, k, ynext);
-                else st_load(seg + (size_t)(t - 1 - t0) * C, w, k, ynext);
-            }
+            if (t > t0) st_load(seg + (size_t)(t - 1 - t0) * C, w, k, ynext);
             const TrK c = coef<SUB>(cc, A.seq, T, p, t, k);
             float ga = 0.f, gt = 0.f;
             // the TR's forward chain once: x1 = rf(yn), x2 = relax_a(x1),
@@ -917,31 +844,21 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
             }
             r_dephase<ND, SUB>(l, k, w);
             // phase(TR - TE) and the readout: input = x3
-            {
-                const Aug<ND> &x = x3;
-                r_phase(l, x, c.pb, c.t2, beta, &gt, gth);
-                float2 cot = {0.f, 0.f};
-                if constexpr (ND == 0) cot = group_pick<SUB>(cv0, cv1, t - t0);
-                if (k == 0) {
-                    if constexpr (ND > 0) {
-                        // dL/dJ_e = 2 sum_d G_de J_d
-                        float2 J[4];
+            r_phase(l, x3, c.pb, c.t2, beta, &gt, gth);
+            if (k == 0) {
+                // dL/dJ_e = 2 sum_d G_de J_d
+                float2 J[4];
 #pragma unroll
-                        for (int d = 0; d < 4; d++) J[d] = x.fp[1 + d];
+                for (int d = 0; d < 4; d++) J[d] = x3.fp[1 + d];
 #pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            float2 cg = {0.f, 0.f};
+                for (int e = 0; e < 4; e++) {
+                    float2 cg = {0.f, 0.f};
 #pragma unroll
-                            for (int d = 0; d < 4; d++) {
-                                const int i = d <= e ? (d * 7 - d * d) / 2 + e : (e * 7 - e * e) / 2 + d;
-                                cg = c_add(cg, c_scale(J[d], 2.f * G[i]));
-                            }
-                            l.fp[1 + e] = c_add(l.fp[1 + e], cg);
-                        }
-                    } else {
-                        // L = Re conj(v) s with s = ro s~: cotangent conj(ro) v
-                        l.fp[0] = c_add(l.fp[0], c_mul(c_conj(c.ro), cot));
+                    for (int d = 0; d < 4; d++) {
+                        const int i = d <= e ? (d * 7 - d * d) / 2 + e : (e * 7 - e * e) / 2 + d;
+                        cg = c_add(cg, c_scale(J[d], 2.f * G[i]));
                     }
+                    l.fp[1 + e] = c_add(l.fp[1 + e], cg);
                 }
             }
             // phase(TE): input = x2
@@ -967,45 +884,23 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
             }
         }
     }
-    if (ND == 0 && A.grad_theta) {
-        // init_rf_pulse's T1/T2 dependence (lane 0)
-        if (k == 0) {
-            const float a0 = (float)(180.0 * M_PI / 180.0f);
-            const float sa0 = sinf(a0), ca0 = cosf(a0);
-            const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
-            const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
-            gth[0] += l.z[0].x * e1d * (ca0 - 1.f);
-            const float2 t2 = {0.f, e2d * sa0};
-            gth[1] += re_dot(l.fp[0], c_mul(c_conj(ro0), c_conj(t2))) +
-                      re_dot(l.fm[0], c_mul(ro0, t2));
-        }
-#pragma unroll
-        for (int d = 0; d < 4; d++) gth[d] = group_sum<SUB>(gth[d]);
-        if (k == 0)
-#pragma unroll
-            for (int d = 0; d < 4; d++) A.grad_theta[a * 4 + d] = gth[d];
-    }
 }
 
 // grad_seq of the atoms of a work-group is summed in LDS (2T floats of
 // dynamic LDS when A.seq_lds): one LDS add per atom and TR, then one global
 // atomic per entry and work-group instead of one per atom.
-// ND = 0: the segment states and the coefficient cache (40 KB; grad_seq
-// goes out per atom, A.seq_part) hold four work-groups per CU, four waves
-// per SIMD at 128 VGPRs
 template <int ND, int SUB>
-__global__ __launch_bounds__(64 * kWaves, ND == 0 ? 4 : 1) void rev_kernel(RevArgs A)
+__global__ __launch_bounds__(64 * kWaves, 1) void rev_kernel(RevArgs A)
 {
     extern __shared__ float gs_lds[];
     __shared__ TrA coef_lds[kCoefLds];
-    __shared__ float2 seg_lds[ND == 0 ? kWaves * kSegLds * 3 * 64 : 1];
     if (A.seq_lds) {
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) gs_lds[i] = 0.f;
         __syncthreads();
     }
     const size_t a = group_atom<SUB>();
     if (a < A.n)
-        rev_atom<ND, SUB>(A, a, gs_lds, coef_lds, seg_lds + (threadIdx.x >> 6) * kSegLds * 3 * 64);
+        rev_atom<ND, SUB>(A, a, gs_lds, coef_lds);
     if (A.seq_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < 2 * A.T; i += blockDim.x) {
@@ -1025,6 +920,478 @@ __global__ __launch_bounds__(256) void seq_reduce_kernel(const float *part, size
     float v = 0.f;
     for (size_t a = blockIdx.y; a < n; a += gridDim.y) v += part[a * T2 + i];
     atomicAdd(&grad_seq[i], v);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (ND = 0) in the lane-group form: an atom on L lanes (8 for w <= 32,
+// 16 for w <= 64), lane j holding EPG states 4j .. 4j + 3, so the per-atom
+// work of a TR (its coefficients, the readout, the sums over states) is
+// shared by L lanes and the four states of a lane are four independent
+// chains.  The forward sweep stores a checkpoint every kAdjSeg = 4 TRs (the
+// atom's 4L states, lanes contiguous, 16-byte pieces); the reverse sweep
+// recomputes a segment's TR-start states and reverses its TRs from them, the
+// intra-TR intermediates recomputed from each TR-start state.  The operators
+// and their adjoints are those of the lane-per-state form above (r_relax,
+// r_dephase, r_phase, r_rf), rearranged:
+//  - states >= w are zero at every TR start, and their adjoints at every
+//    step, so the truncation of dephase_gradients reduces to F+_w <- 0
+//    (folded into the relaxation that follows the shift) and, in reverse,
+//    l-_{w-1} <- 0 before the shift (folded into its relaxation's scaling);
+//    the values that cross a group boundary in the shifts are replaced by
+//    selects, never by products, so the lanes of another atom cannot leak in;
+//  - the forward step applies relax(TE), phase(TE) and phase(TR - TE) as one
+//    complex factor e2a e^{i beta TE} e^{i beta (TR - TE)} (times the frame
+//    change) on F+ and its conjugate on F-;
+//  - the phase adjoints take the operator's output y: dL/dt of y = P x is
+//    Re <l, i beta y> on F+ (minus on F-), so phase(TR - TE)'s term is read off
+//    the shifted state that the relaxation adjoint already holds (the shift
+//    maps (i y+, -i y-) onto itself) and phase(TE)'s off its output.
+constexpr int kAdjM = 4;     // states per lane
+constexpr int kAdjSeg = 4;   // TRs per checkpoint segment
+
+struct AState {
+    float2 fp[kAdjM], fm[kAdjM], z[kAdjM];
+};
+
+// the lane's place in its group and in the state range
+struct AdjLane {
+    int j;
+    bool first, last;        // j == 0, j == L - 1
+    bool atw[kAdjM];         // state 4j + i == w
+    bool atw1[kAdjM];        // state 4j + i == w - 1
+};
+
+// per-TR coefficients of one atom (tr_coef), 64 bytes
+struct alignas(16) AdjCoef {
+    float sa, ca, e1a, e2a;
+    float e1b, e2b, al, te;
+    float2 pa, pb;
+    float2 ro;
+    float t2, pad;
+};
+static_assert(sizeof(AdjCoef) == 64, "AdjCoef layout");
+
+__device__ __forceinline__ AdjCoef adj_coef_of(const DTr &q, const Atom &p)
+{
+    const TrK c = tr_coef(q, p);
+    AdjCoef e;
+    e.sa = c.sa;
+    e.ca = c.ca;
+    e.e1a = c.e1a;
+    e.e2a = c.e2a;
+    e.e1b = c.e1b;
+    e.e2b = c.e2b;
+    e.al = c.al;
+    e.te = c.te;
+    e.pa = c.pa;
+    e.pb = c.pb;
+    e.ro = c.ro;
+    e.t2 = c.t2;
+    e.pad = 0.f;
+    return e;
+}
+
+// The coefficients of the group's atom, cached L TRs at a time in LDS: lane
+// j computes TR b L + j, the group reads each TR's entry back (segments of
+// kAdjSeg TRs never straddle a block).  All groups of a wave walk the same
+// TRs, so a block changes for the whole wave at once.
+template <int L>
+struct AdjCache {
+    AdjCoef *blk;   // the group's L entries
+    int cur;        // block held, -1: none
+};
+
+template <int L>
+__device__ __forceinline__ AdjCoef adj_coef(AdjCache<L> &cc, const DTr *seq, int T, const Atom &p,
+                                            int t, int j)
+{
+    const int b = t / L;
+    if (b != cc.cur) {
+        wave_sync();
+        const int tt = b * L + j;
+        if (tt < T) cc.blk[j] = adj_coef_of(seq[tt], p);
+        cc.cur = b;
+        wave_sync();
+    }
+    return cc.blk[t - b * L];
+}
+
+// Lane i of a 16-lane DPP row receives lane i - 1 (row_shr:1) or i + 1
+// (row_shl:1); the row's end lanes receive 0.  Groups of 8 or 16 lanes never
+// straddle a row.
+__device__ __forceinline__ float row_shr1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x111, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float row_shl1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x101, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float2 row_shr1(float2 v) { return {row_shr1(v.x), row_shr1(v.y)}; }
+__device__ __forceinline__ float2 row_shl1(float2 v) { return {row_shl1(v.x), row_shl1(v.y)}; }
+
+// sum over the L lanes of the group (8: quads, then half rows; 16: rows)
+template <int L>
+__device__ __forceinline__ float adj_gsum(float v)
+{
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    v = dpp_add<0x141>(v);
+    if (L == 16) v = dpp_add<0x140>(v);
+    return v;
+}
+
+__device__ __forceinline__ float re_idot(float2 l, float2 y) { return re_dot(l, c_i(y)); }
+
+__device__ __forceinline__ void a_rf(AState &y, float sa, float ca)
+{
+    const C4 m = rf_c(sa, ca);
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) rf3<true>(m, y.fp[i], y.fm[i], y.z[i], y.fp[i], y.fm[i], y.z[i]);
+}
+
+// relax over an interval with F+ <- P F+, F- <- conj(P) F- (P = e2 times the
+// precession factors), Z <- e1 Z + (1 - e1) on state 0
+__device__ __forceinline__ void a_decay(AState &y, float2 P, float e1, const AdjLane &ln)
+{
+    const float2 Q = c_conj(P);
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        y.fp[i] = c_mul(P, y.fp[i]);
+        y.fm[i] = c_mul(Q, y.fm[i]);
+        y.z[i] = c_scale(y.z[i], e1);
+    }
+    if (ln.first) y.z[0].x += 1.f - e1;
+}
+
+// dephase_gradients (functions.cu:619-668): F+_k <- F+_{k-1}, F-_k <- F-_{k+1},
+// F+_0 <- conj(F-_0); F+_w is left for the caller (states >= w are zero).
+__device__ __forceinline__ void a_shift(AState &y, const AdjLane &ln)
+{
+    const float2 nxt = row_shl1(y.fm[0]);            // F-_{4j+4}
+    const float2 prv = row_shr1(y.fp[kAdjM - 1]);    // F+_{4j-1}
+#pragma unroll
+    for (int i = 0; i < kAdjM - 1; i++) y.fm[i] = y.fm[i + 1];
+    y.fm[kAdjM - 1] = ln.last ? float2{0.f, 0.f} : nxt;
+#pragma unroll
+    for (int i = kAdjM - 1; i > 0; i--) y.fp[i] = y.fp[i - 1];
+    y.fp[0] = ln.first ? c_conj(y.fm[0]) : prv;
+}
+
+// one TR from its TR-start state to the next
+__device__ __forceinline__ void a_tr(AState &y, const AdjCoef &c, const AdjLane &ln)
+{
+    a_rf(y, c.sa, c.ca);
+    a_decay(y, c_scale(c_mul(c.pa, c.pb), c.e2a), c.e1a, ln);
+    a_shift(y, ln);
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        y.fp[i] = c_scale(y.fp[i], ln.atw[i] ? 0.f : c.e2b);   // F+_w <- 0
+        y.fm[i] = c_scale(y.fm[i], c.e2b);
+        y.z[i] = c_scale(y.z[i], c.e1b);
+    }
+    if (ln.first) y.z[0].x += 1.f - c.e1b;
+}
+
+// init_rf_pulse(180 deg, 0 deg, TI) in the frame of TR 0 (f_init)
+__device__ __forceinline__ void a_init(AState &y, const Atom &p, const AdjLane &ln, float2 ro0)
+{
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) y.fp[i] = y.fm[i] = y.z[i] = {0.f, 0.f};
+    if (!ln.first) return;
+    const float a0 = (float)(180.0 * M_PI / 180.0f);
+    const float sa0 = sinf(a0), ca0 = cosf(a0);
+    const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
+    const float2 tmp = {0.f, e2 * sa0};
+    y.fp[0] = c_mul(c_conj(ro0), c_conj(tmp));
+    y.fm[0] = c_mul(ro0, tmp);
+    y.z[0] = {e1 * ca0 + 1.f - e1, 0.f};
+}
+
+// adjoint of relax (r_relax) over an interval t with input x: *gt (if
+// given) receives dL/dt, *ub the derivative Re <l, (i x+, -i x-)> that a
+// phase step whose output was shifted into x contributes (scaled by the
+// relaxation, as the adjoint there carries it).  MASK: l-_{w-1} <- 0.
+template <bool MASK>
+__device__ __forceinline__ void a_rrelax(AState &l, const AState &x, float e1, float e2, float t,
+                                         const Atom &p, const AdjLane &ln, float gth[4], float *gt,
+                                         float *ub)
+{
+    const float i1 = p.iT1, i2 = p.iT2;
+    float sf = 0.f, sz = 0.f, u = 0.f;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        sf += re_dot(l.fp[i], x.fp[i]) + re_dot(l.fm[i], x.fm[i]);
+        sz += re_dot(l.z[i], x.z[i]);
+        if (ub) u += re_idot(l.fp[i], x.fp[i]) - re_idot(l.fm[i], x.fm[i]);
+    }
+    if (ln.first) sz -= l.z[0].x;   // the source term: Z_0 - 1
+    gth[0] += sz * (e1 * t * i1 * i1);
+    gth[1] += sf * (e2 * t * i2 * i2);
+    if (gt) *gt += sf * (-e2 * i2) + sz * (-e1 * i1);
+    if (ub) *ub = u * e2;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        l.fp[i] = c_scale(l.fp[i], e2);
+        l.fm[i] = c_scale(l.fm[i], (MASK && ln.atw1[i]) ? 0.f : e2);
+        l.z[i] = c_scale(l.z[i], e1);
+    }
+}
+
+// adjoint of the shift (r_dephase)
+__device__ __forceinline__ void a_rshift(AState &l, const AdjLane &ln)
+{
+    const float2 nxt = row_shl1(l.fp[0]);            // l+_{4j+4}
+    const float2 prv = row_shr1(l.fm[kAdjM - 1]);    // l-_{4j-1}
+    const float2 lp0 = l.fp[0];                      // l+_0 on lane 0
+#pragma unroll
+    for (int i = 0; i < kAdjM - 1; i++) l.fp[i] = l.fp[i + 1];
+    l.fp[kAdjM - 1] = ln.last ? float2{0.f, 0.f} : nxt;
+#pragma unroll
+    for (int i = kAdjM - 1; i > 0; i--) l.fm[i] = l.fm[i - 1];
+    l.fm[0] = ln.first ? float2{0.f, 0.f} : prv;
+    if (ln.first) l.fm[1] = c_add(l.fm[1], c_conj(lp0));
+}
+
+// adjoint of a phase step: l+ <- conj(P) l+, l- <- P l-
+__device__ __forceinline__ void a_rphase(AState &l, float2 P)
+{
+    const float2 Q = c_conj(P);
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        l.fp[i] = c_mul(Q, l.fp[i]);
+        l.fm[i] = c_mul(P, l.fm[i]);
+    }
+}
+
+// adjoint of the RF (r_rf): x the operator's input
+__device__ __forceinline__ void a_rrf(AState &l, const AState &x, const AdjCoef &c, float B1,
+                                      float *ga, float gth[4])
+{
+    const C4 m = rf_c(c.sa, c.ca), m1 = rf_d1(c.sa, c.ca);
+    float g_a = 0.f;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        float2 a, b, d;
+        rf3<false>(m1, x.fp[i], x.fm[i], x.z[i], a, b, d);
+        g_a += re_dot(l.fp[i], a) + re_dot(l.fm[i], b) + re_dot(l.z[i], d);
+        rf3_adj<true>(m, l.fp[i], l.fm[i], l.z[i], l.fp[i], l.fm[i], l.z[i]);
+    }
+    *ga += B1 * g_a;
+    gth[3] += c.al * g_a;
+}
+
+// the reverse of TR n from its TR-start state yn; cot = dL/ds_n
+__device__ __forceinline__ void a_rev_tr(AState &l, const AState &yn, const AdjCoef &c,
+                                         const Atom &p, float beta, const AdjLane &ln, float2 cot,
+                                         float &ga, float &gt, float gth[4])
+{
+    AState x3 = yn;   // x3 = phase_a(relax_a(rf(yn))): the readout state
+    a_rf(x3, c.sa, c.ca);
+    a_decay(x3, c_scale(c.pa, c.e2a), c.e1a, ln);
+    float ub;
+    {
+        // relax(TR - TE) over x = dephase(phase_b(x3)); phase(TR - TE)'s
+        // derivative read off x
+        AState x = x3;
+        const float2 Q = c_conj(c.pb);
+#pragma unroll
+        for (int i = 0; i < kAdjM; i++) {
+            x.fp[i] = c_mul(c.pb, x.fp[i]);
+            x.fm[i] = c_mul(Q, x.fm[i]);
+        }
+        a_shift(x, ln);
+        a_rrelax<true>(l, x, c.e1b, c.e2b, c.t2, p, ln, gth, &gt, &ub);
+    }
+    a_rshift(l, ln);
+    gt += beta * ub;
+    gth[2] += kCB0 * c.t2 * ub;
+    a_rphase(l, c.pb);
+    // the readout s = ro F+_0: L = Re conj(v) s gives the cotangent conj(ro) v
+    if (ln.first) l.fp[0] = c_add(l.fp[0], c_mul(c_conj(c.ro), cot));
+    {
+        // phase(TE), output x3
+        float u = 0.f;
+#pragma unroll
+        for (int i = 0; i < kAdjM; i++) u += re_idot(l.fp[i], x3.fp[i]) - re_idot(l.fm[i], x3.fm[i]);
+        gth[2] += kCB0 * c.te * u;
+        a_rphase(l, c.pa);
+    }
+    AState x1 = yn;   // relax(TE) over x1 = rf(yn)
+    a_rf(x1, c.sa, c.ca);
+    a_rrelax<false>(l, x1, c.e1a, c.e2a, c.te, p, ln, gth, nullptr, nullptr);
+    a_rrf(l, yn, c, p.B1, &ga, gth);
+}
+
+// a group's states as 6 pieces of 16 bytes per lane, lanes contiguous per
+// piece (stride S pieces)
+template <int S>
+__device__ __forceinline__ void a_store(float4 *c, const AState &y)
+{
+    c[0 * S] = make_float4(y.fp[0].x, y.fp[0].y, y.fp[1].x, y.fp[1].y);
+    c[1 * S] = make_float4(y.fp[2].x, y.fp[2].y, y.fp[3].x, y.fp[3].y);
+    c[2 * S] = make_float4(y.fm[0].x, y.fm[0].y, y.fm[1].x, y.fm[1].y);
+    c[3 * S] = make_float4(y.fm[2].x, y.fm[2].y, y.fm[3].x, y.fm[3].y);
+    c[4 * S] = make_float4(y.z[0].x, y.z[0].y, y.z[1].x, y.z[1].y);
+    c[5 * S] = make_float4(y.z[2].x, y.z[2].y, y.z[3].x, y.z[3].y);
+}
+
+template <int S>
+__device__ __forceinline__ void a_load(const float4 *c, AState &y)
+{
+    float4 v;
+    v = c[0 * S]; y.fp[0] = {v.x, v.y}; y.fp[1] = {v.z, v.w};
+    v = c[1 * S]; y.fp[2] = {v.x, v.y}; y.fp[3] = {v.z, v.w};
+    v = c[2 * S]; y.fm[0] = {v.x, v.y}; y.fm[1] = {v.z, v.w};
+    v = c[3 * S]; y.fm[2] = {v.x, v.y}; y.fm[3] = {v.z, v.w};
+    v = c[4 * S]; y.z[0] = {v.x, v.y}; y.z[1] = {v.z, v.w};
+    v = c[5 * S]; y.z[2] = {v.x, v.y}; y.z[3] = {v.z, v.w};
+}
+
+// checkpoint floats per atom of adj_kernel<L>
+size_t adj_ck_floats(int T, int L)
+{
+    return (size_t)((T + kAdjSeg - 1) / kAdjSeg) * 6 * 4 * L;
+}
+
+// One wave per work-group (64 / L atoms; small work-groups keep the last
+// round of a launch short), two waves per SIMD (256 registers, 16 KB of LDS
+// per wave): the segment's first and last TR-start states, the adjoint, the
+// next segment's checkpoint and the intermediates of a TR in registers, the
+// segment's middle TR-start states in LDS (the lane's own slots).
+template <int L>
+__global__ __launch_bounds__(64, 2) void adj_kernel(RevArgs A)
+{
+    static_assert(L == 8 || L == 16, "8 or 16 lanes per atom");
+    static_assert(L % kAdjSeg == 0, "segments must not straddle a coefficient block");
+    static_assert(kAdjSeg == 4, "the segment loop holds TR-start states 1, 2 in LDS");
+    __shared__ AdjCoef coef_lds[64];
+    __shared__ float4 seg_lds[2 * 6 * 64];
+    const int lane = threadIdx.x;
+    const size_t ag = (size_t)blockIdx.x * (64 / L) + lane / L;
+    // lanes of a missing atom run on the last one and store nothing, so that
+    // every lane of a row takes part in the shifts and sums
+    const bool valid = ag < A.n;
+    const size_t a = valid ? ag : A.n - 1;
+    const int T = A.T, w = A.w;
+    AdjLane ln;
+    ln.j = lane & (L - 1);
+    ln.first = ln.j == 0;
+    ln.last = ln.j == L - 1;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        ln.atw[i] = kAdjM * ln.j + i == w;
+        ln.atw1[i] = kAdjM * ln.j + i == w - 1;
+    }
+    const int j = ln.j;
+    const Atom p = load_atom(A.theta, a);
+    AdjCache<L> cc{coef_lds + (lane & ~(L - 1)), -1};
+    const int nck = (T + kAdjSeg - 1) / kAdjSeg;
+    float4 *ck = reinterpret_cast<float4 *>(A.scratch) + a * (size_t)nck * 6 * L + j;
+
+    // forward sweep: a checkpoint every kAdjSeg TRs
+    const float2 ro0 = frame0(A.seq);
+    AState y;
+    a_init(y, p, ln, ro0);
+    for (int t = 0; t < T; t++) {
+        if (t % kAdjSeg == 0 && valid) a_store<L>(ck + (size_t)(t / kAdjSeg) * 6 * L, y);
+        a_tr(y, adj_coef<L>(cc, A.seq, T, p, t, j), ln);
+    }
+
+    // reverse sweep, segment by segment; the checkpoint and the cotangents
+    // of the segment before (in time) are loaded while a segment is reversed
+    AState l;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) l.fp[i] = l.fm[i] = l.z[i] = {0.f, 0.f};
+    float gth[4] = {0.f, 0.f, 0.f, 0.f};
+    const float beta = kCB0 * p.B0;
+    const float2 *cr = A.cot + a * (size_t)T;
+    AState ckn;
+    float2 cvn;
+    auto seg_in = [&](int sg) {
+        a_load<L>(ck + (size_t)sg * 6 * L, ckn);
+        const int tt = sg * kAdjSeg + j;
+        cvn = (j < kAdjSeg && tt < T) ? cr[tt] : float2{0.f, 0.f};
+    };
+    const int gbase = lane & ~(L - 1);
+    float4 *slot = seg_lds + lane;   // slot s: TR-start state s + 1
+    auto rev_one = [&](int t, int s, const AState &yn, float2 cv) {
+        const AdjCoef c = adj_coef<L>(cc, A.seq, T, p, t, j);
+        const float2 cot = {__shfl(cv.x, gbase + s), __shfl(cv.y, gbase + s)};
+        float ga = 0.f, gt = 0.f;
+        a_rev_tr(l, yn, c, p, beta, ln, cot, ga, gt, gth);
+        if (A.seq_part) {
+            ga = adj_gsum<L>(ga);
+            gt = adj_gsum<L>(gt);
+            if (ln.first && valid) {
+                A.seq_part[a * 2 * T + t] = ga * A.fa_unit;
+                A.seq_part[a * 2 * T + T + t] = gt;
+            }
+        }
+    };
+    seg_in(nck - 1);
+    int sg = nck - 1;
+    const int tail = T - sg * kAdjSeg;   // 1 .. kAdjSeg TRs in the last segment
+    if (tail < kAdjSeg) {
+        const int t0 = sg * kAdjSeg;
+        const AState s0 = ckn;
+        const float2 cv = cvn;
+        if (sg > 0) seg_in(sg - 1);
+        AState yy = s0;
+#pragma unroll 1
+        for (int s = 1; s < tail; s++) {
+            a_tr(yy, adj_coef<L>(cc, A.seq, T, p, t0 + s - 1, j), ln);
+            a_store<64>(slot + (s - 1) * 6 * 64, yy);
+        }
+#pragma unroll 1
+        for (int s = tail - 1; s >= 1; s--) {
+            AState yn;
+            a_load<64>(slot + (s - 1) * 6 * 64, yn);
+            rev_one(t0 + s, s, yn, cv);
+        }
+        rev_one(t0, 0, s0, cv);
+        sg--;
+    }
+    for (; sg >= 0; sg--) {
+        const int t0 = sg * kAdjSeg;
+        const AState s0 = ckn;
+        const float2 cv = cvn;
+        if (sg > 0) seg_in(sg - 1);
+        AState yy = s0;
+        a_tr(yy, adj_coef<L>(cc, A.seq, T, p, t0, j), ln);
+        a_store<64>(slot, yy);
+        a_tr(yy, adj_coef<L>(cc, A.seq, T, p, t0 + 1, j), ln);
+        a_store<64>(slot + 6 * 64, yy);
+        a_tr(yy, adj_coef<L>(cc, A.seq, T, p, t0 + 2, j), ln);
+        rev_one(t0 + 3, 3, yy, cv);
+#pragma unroll 1
+        for (int s = 2; s >= 1; s--) {
+            AState yn;
+            a_load<64>(slot + (s - 1) * 6 * 64, yn);
+            rev_one(t0 + s, s, yn, cv);
+        }
+        rev_one(t0, 0, s0, cv);
+    }
+    if (A.grad_theta) {
+        // init_rf_pulse's T1/T2 dependence (state 0, lane 0)
+        if (ln.first) {
+            const float a0 = (float)(180.0 * M_PI / 180.0f);
+            const float sa0 = sinf(a0), ca0 = cosf(a0);
+            const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
+            const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
+            gth[0] += l.z[0].x * e1d * (ca0 - 1.f);
+            const float2 t2 = {0.f, e2d * sa0};
+            gth[1] += re_dot(l.fp[0], c_mul(c_conj(ro0), c_conj(t2))) +
+                      re_dot(l.fm[0], c_mul(ro0, t2));
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) gth[d] = adj_gsum<L>(gth[d]);
+        if (ln.first && valid)
+#pragma unroll
+            for (int d = 0; d < 4; d++) A.grad_theta[a * 4 + d] = gth[d];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1327,7 +1694,7 @@ dim3 groups(size_t n, int sub)
 size_t diff_scratch_floats(int nd, int T, int w)
 {
     // ND = 0: checkpoints, then the atom's grad_seq partials (2T)
-    if (nd == 0) return (size_t)((T + kSegLds - 1) / kSegLds) * 6 * w + (size_t)2 * T;
+    if (nd == 0) return adj_ck_floats(T, w <= 32 ? 8 : 16) + (size_t)2 * T;
     const int nck = (T + kCkpt - 1) / kCkpt;
     return (size_t)(nck + kCkpt) * 6 * (1 + nd) * w;
 }
@@ -1352,7 +1719,8 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.seq = seq;
     A.T = T;
     A.w = w;
-    A.nck = (T + kSegLds - 1) / kSegLds;
+    const int L = w <= 32 ? 8 : 16;
+    A.nck = (T + kAdjSeg - 1) / kAdjSeg;
     A.theta = theta;
     A.n = n;
     A.cot = cot;
@@ -1361,11 +1729,11 @@ int launch_adjoint(const DTr *seq, int T, int w, const float *theta, size_t n, c
     A.fa_unit = fa_unit;
     A.scratch = scratch;
     // the checkpoints of all n atoms, then their grad_seq partials
-    const size_t ck_floats = (size_t)A.nck * 6 * w;
-    A.seq_part = grad_seq ? scratch + n * ck_floats : nullptr;
-    const size_t lds = rev_lds_bytes(A);
-    if (w <= 32) rev_kernel<0, 32><<<groups(n, 32), 64 * kWaves, lds, s>>>(A);
-    else rev_kernel<0, 64><<<groups(n, 64), 64 * kWaves, lds, s>>>(A);
+    A.seq_part = grad_seq ? scratch + n * adj_ck_floats(T, L) : nullptr;
+    const size_t per_block = 64 / L;
+    const unsigned nb = (unsigned)((n + per_block - 1) / per_block);
+    if (L == 8) adj_kernel<8><<<nb, 64, 0, s>>>(A);
+    else adj_kernel<16><<<nb, 64, 0, s>>>(A);
     if (grad_seq) {
         const unsigned gy = (unsigned)(n < 64 ? n : 64);
         seq_reduce_kernel<<<dim3((unsigned)((2 * T + 255) / 256), gy), 256, 0, s>>>(
