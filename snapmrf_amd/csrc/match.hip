@@ -63,6 +63,7 @@ struct MatchArgs {
     size_t row_stride;          // atom rows per slice in `atoms`
     size_t gidx_stride;         // global atom index step per slice
     size_t gidx_base;           // global index of atom 0 of slice 0
+    size_t sig_bytes;           // bytes of `sig` (V T complex values)
     int T;
 };
 
@@ -93,7 +94,29 @@ __device__ __forceinline__ float4 load_pair(const float2 *row, int t, int T, boo
 // SIMD; 2 % faster than four waves of 64 x 64 at one per SIMD).
 constexpr int kFp32AtomGroups = 4;
 constexpr int kFp32Threads = 128 * kFp32AtomGroups;
+constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
+constexpr uint32_t kOob = 0x80000000u;         // a buffer offset past every resource used here
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, size_t bytes)
+{
+    const uint32_t n = bytes > 0xffffffffu ? 0xffffffffu : (uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)n,
+                                             (int)kRsrcWord3);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// DMA = true (even T, signals below 2 GB): the chunks go from global memory
+// straight into LDS (buffer_load ... lds), no registers or LDS stores on the
+// way.  A wave's copy of 64 x 16 bytes lands as one 1 KB block, lane l at
+// byte 16 l, so the LDS holds a chunk fragment-major: block (operand, row
+// block rb of 16 rows, half q of the chunk) has lane 16 g + r holding row
+// 16 rb + r, timepoints 8 q + 2 g and + 1 -- the 16 bytes that lane (g, r)
+// of an MFMA fragment reads, so fragment reads are ds_read_b128 at 16 l.
+// Rows outside the sub-tile or tile and timepoints past T get an offset past
+// the resource, which the range check turns into zeros.
+// DMA = false: the chunks are staged through registers into padded rows.
+template <bool DMA>
 __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
 {
     constexpr int NWA = kFp32AtomGroups;
@@ -101,6 +124,9 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
     constexpr int NIB = AT / NWA / 16;  // 16-atom MFMA blocks per wave
     constexpr int RP = NT / 8;          // staged rows per pass
     constexpr int NP = AT / RP;         // passes (atoms and voxels: 128 rows each)
+    constexpr int NRB = AT / 16;        // 16-row blocks per operand (AT == VT)
+    constexpr int NBW = 2 * NRB * 2 / (NT / 64);   // DMA blocks per wave and chunk
+    static_assert(AT == VT, "one block layout for both operands");
     const bool even_t = (a.T & 1) == 0;
     const int tile_id = blockIdx.x;
     if (tile_id >= *a.ntiles) return;
@@ -112,14 +138,18 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
     const size_t idx_base = a.gidx_base + (size_t)tile.slice * a.gidx_stride;
 
     // two LDS buffers: chunk c + 1 is stored while chunk c is multiplied
-    __shared__ __attribute__((aligned(16))) float lds_a[2][AT * LS];
-    __shared__ __attribute__((aligned(16))) float lds_b[2][VT * LS];
+    __shared__ __attribute__((aligned(16))) float lds_a[DMA ? 1 : 2][DMA ? 4 : AT * LS];
+    __shared__ __attribute__((aligned(16))) float lds_b[DMA ? 1 : 2][DMA ? 4 : VT * LS];
+    // DMA: two buffers (separate arrays, so the compiler sees that copies
+    // into one do not touch reads of the other), [operand][row block][half][lane]
+    __shared__ floatx4 lds_d0[DMA ? 2 * NRB * 2 * 64 : 1];
+    __shared__ floatx4 lds_d1[DMA ? 2 * NRB * 2 * 64 : 1];
     __shared__ float red_s[NWA][4][VT];
     __shared__ uint32_t red_i[NWA][4][VT];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: DMA operands
     const int wa = wave % NWA;        // atom group of the sub-tile (AT / NWA atoms)
     const int wv = wave / NWA;        // voxel half of the tile (64 voxels)
     const int g = lane >> 4;          // k group
@@ -132,11 +162,29 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
 
     // voxel rows of this tile (gathered through the permutation)
     const float2 *vrow[NP];
+    // DMA: the wave's blocks w NBW .. w NBW + NBW - 1 (block b: operand
+    // b / (2 NRB), row block (b / 2) % NRB, half b % 2); the lane's row byte
+    // offset in its operand's resource, kOob for a row outside
+    uint32_t drow[NBW];
+    const __amdgpu_buffer_rsrc_t rsig = make_rsrc(a.sig, DMA ? a.sig_bytes : 0);
+    if constexpr (DMA) {
 #pragma unroll
-    for (int p = 0; p < NP; p++) {
-        const int vr = st_row + RP * p;
-        vrow[p] = (vr < tile.count) ? a.sig + (size_t)a.perm[tile.pstart + vr] * (size_t)a.T
-                                    : nullptr;
+        for (int j = 0; j < NBW; j++) {
+            const int b = wave * NBW + j;
+            const int row = ((b >> 1) % NRB) * 16 + r16;
+            if (b >= 2 * NRB) {   // voxels
+                drow[j] = row < tile.count
+                              ? (uint32_t)a.perm[tile.pstart + row] * (uint32_t)a.T * 8u
+                              : kOob;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int vr = st_row + RP * p;
+            vrow[p] = (vr < tile.count) ? a.sig + (size_t)a.perm[tile.pstart + vr] * (size_t)a.T
+                                        : nullptr;
+        }
     }
 
     float best[4];
@@ -149,12 +197,36 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
 
     const int nchunks = (a.T + KC - 1) / KC;
     for (size_t sub = a_begin; sub < a_end; sub += AT) {
+        const size_t nsub = (a_end - sub < (size_t)AT) ? a_end - sub : (size_t)AT;
         const float2 *arow[NP];
+        __amdgpu_buffer_rsrc_t ratom;
+        if constexpr (DMA) {
+            ratom = make_rsrc(a.atoms + (row_base + sub) * (size_t)a.T, nsub * (size_t)a.T * 8);
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            const size_t ar = sub + st_row + RP * p;
-            arow[p] = (ar < a_end) ? a.atoms + (row_base + ar) * (size_t)a.T : nullptr;
+            for (int j = 0; j < NBW; j++) {
+                const int b = wave * NBW + j;
+                const int row = ((b >> 1) % NRB) * 16 + r16;
+                if (b < 2 * NRB) drow[j] = row < (int)nsub ? (uint32_t)row * (uint32_t)a.T * 8u : kOob;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const size_t ar = sub + st_row + RP * p;
+                arow[p] = (ar < a_end) ? a.atoms + (row_base + ar) * (size_t)a.T : nullptr;
+            }
         }
+        // DMA: issue chunk c's copies into buffer d
+        auto dma_chunk = [&](int c, floatx4 *d) {
+#pragma unroll
+            for (int j = 0; j < NBW; j++) {
+                const int b = wave * NBW + j;
+                const int t = c * KC + (b & 1) * 8 + 2 * g;
+                const uint32_t off = (drow[j] != kOob && t < a.T) ? drow[j] + (uint32_t)t * 8u : kOob;
+                lds_void *dst = (lds_void *)&d[b * 64];
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(b < 2 * NRB ? ratom : rsig, dst, 16,
+                                                         (int)off, 0, 0, 0);
+            }
+        };
         // three real products (Gauss): P1 = ar.xr, P2 = ai.xi,
         // P3 = (ar - ai).(xr + xi); Re S = P1 + P2, Im S = P3 - P1 + P2
         floatx4 acc_p1[NIB][4], acc_p2[NIB][4], acc_p3[NIB][4];
@@ -172,16 +244,20 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
         // stored into the other LDS buffer after three quarters of them (one
         // barrier per chunk)
         float4 ga[NP], gb[NP];
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-            ga[p] = load_pair(arow[p], st_col >> 1, a.T, even_t);
-            gb[p] = load_pair(vrow[p], st_col >> 1, a.T, even_t);
-        }
         // (the previous sub-tile's reads ended at its last chunk's barrier)
+        if constexpr (DMA) {
+            dma_chunk(0, lds_d0);
+        } else {
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            *reinterpret_cast<float4 *>(&lds_a[0][(st_row + RP * p) * LS + st_col]) = ga[p];
-            *reinterpret_cast<float4 *>(&lds_b[0][(st_row + RP * p) * LS + st_col]) = gb[p];
+            for (int p = 0; p < NP; p++) {
+                ga[p] = load_pair(arow[p], st_col >> 1, a.T, even_t);
+                gb[p] = load_pair(vrow[p], st_col >> 1, a.T, even_t);
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                *reinterpret_cast<float4 *>(&lds_a[0][(st_row + RP * p) * LS + st_col]) = ga[p];
+                *reinterpret_cast<float4 *>(&lds_b[0][(st_row + RP * p) * LS + st_col]) = gb[p];
+            }
         }
         __syncthreads();
         // LDS fragments are double-buffered in registers: the reads for the
@@ -191,15 +267,26 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
         // are read while the second half runs.  Every read of a buffer is
         // issued before the barrier that precedes the next store into it.
         floatx4 fa[2][NIB], fx[2][4];
-        auto read_frags = [&](int b, int q, floatx4 (&ra)[NIB], floatx4 (&rx)[4]) {
+        // buffer b (DMA: d)
+        auto read_frags = [&](int b, const floatx4 *d, int q, floatx4 (&ra)[NIB],
+                              floatx4 (&rx)[4]) {
+            if constexpr (DMA) {
 #pragma unroll
-            for (int i = 0; i < NIB; i++)
-                ra[i] = *reinterpret_cast<const floatx4 *>(
-                    &lds_a[b][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
+                for (int i = 0; i < NIB; i++)
+                    ra[i] = d[((0 * NRB + wa * NIB + i) * 2 + q) * 64 + lane];
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                rx[v] = *reinterpret_cast<const floatx4 *>(
-                    &lds_b[b][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
+                for (int v = 0; v < 4; v++)
+                    rx[v] = d[((1 * NRB + wv * 4 + v) * 2 + q) * 64 + lane];
+            } else {
+#pragma unroll
+                for (int i = 0; i < NIB; i++)
+                    ra[i] = *reinterpret_cast<const floatx4 *>(
+                        &lds_a[b][(wa * (AT / NWA) + i * 16 + r16) * LS + q * 16 + g * 4]);
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    rx[v] = *reinterpret_cast<const floatx4 *>(
+                        &lds_b[b][(wv * 64 + v * 16 + r16) * LS + q * 16 + g * 4]);
+            }
         };
         // MFMA step u of a fragment set: timepoint u of every lane group
         auto mfma_step = [&](const floatx4 (&ra)[NIB], const floatx4 (&rx)[4], int u) {
@@ -221,36 +308,51 @@ __global__ __launch_bounds__(kFp32Threads) void match_fp32_kernel(MatchArgs a)
                 }
         };
         static_assert(KF == 32, "two 8-timepoint halves per chunk");
-        read_frags(0, 0, fa[0], fx[0]);
-        for (int c = 0; c < nchunks; c++) {
-            const int cb = c & 1;
+        read_frags(0, lds_d0, 0, fa[0], fx[0]);
+        // chunk c from buffer cb (DMA: cur), the next into the other (nxt)
+        auto chunk = [&](int c, int cb, floatx4 *cur, floatx4 *nxt) {
             const bool more = c + 1 < nchunks;
             if (more) {
-                const int tc = (c + 1) * KC + (st_col >> 1);
+                if constexpr (DMA) {
+                    // nxt was last read in chunk c - 1, before the barrier
+                    // that ended it
+                    dma_chunk(c + 1, nxt);
+                } else {
+                    const int tc = (c + 1) * KC + (st_col >> 1);
 #pragma unroll
-                for (int p = 0; p < NP; p++) {
-                    ga[p] = load_pair(arow[p], tc, a.T, even_t);
-                    gb[p] = load_pair(vrow[p], tc, a.T, even_t);
+                    for (int p = 0; p < NP; p++) {
+                        ga[p] = load_pair(arow[p], tc, a.T, even_t);
+                        gb[p] = load_pair(vrow[p], tc, a.T, even_t);
+                    }
                 }
             }
-            read_frags(cb, 1, fa[1], fx[1]);
+            read_frags(cb, cur, 1, fa[1], fx[1]);
             __builtin_amdgcn_sched_barrier(0);
             mfma_step(fa[0], fx[0], 0);
             mfma_step(fa[0], fx[0], 1);
             mfma_step(fa[1], fx[1], 0);
             __builtin_amdgcn_sched_barrier(0);
-            if (more) {
+            if constexpr (!DMA) {
+                if (more) {
 #pragma unroll
-                for (int p = 0; p < NP; p++) {
-                    *reinterpret_cast<float4 *>(&lds_a[cb ^ 1][(st_row + RP * p) * LS + st_col]) = ga[p];
-                    *reinterpret_cast<float4 *>(&lds_b[cb ^ 1][(st_row + RP * p) * LS + st_col]) = gb[p];
+                    for (int p = 0; p < NP; p++) {
+                        *reinterpret_cast<float4 *>(&lds_a[cb ^ 1][(st_row + RP * p) * LS + st_col]) = ga[p];
+                        *reinterpret_cast<float4 *>(&lds_b[cb ^ 1][(st_row + RP * p) * LS + st_col]) = gb[p];
+                    }
                 }
             }
             __syncthreads();
-            if (more) read_frags(cb ^ 1, 0, fa[0], fx[0]);
+            if (more) read_frags(cb ^ 1, nxt, 0, fa[0], fx[0]);
             __builtin_amdgcn_sched_barrier(0);
             mfma_step(fa[1], fx[1], 1);
+        };
+        // two chunks per iteration: the buffers are fixed in each copy
+        int c = 0;
+        for (; c + 1 < nchunks; c += 2) {
+            chunk(c, 0, lds_d0, lds_d1);
+            chunk(c + 1, 1, lds_d1, lds_d0);
         }
+        if (c < nchunks) chunk(c, 0, lds_d0, lds_d1);
         // epilogue: D[row = 4g + r][col = r16] of each 16x16 tile
 #pragma unroll
         for (int i = 0; i < NIB; i++)
@@ -463,7 +565,6 @@ constexpr int LVT = 256;          // voxels per work-group tile (64 per wave)
 // lane lists make its 4*LK candidates per atom chunk.
 constexpr int kMaxChunks = 32;    // atom chunks per voxel tile (rescore merges their lists)
 constexpr int kMinChunkAtoms = 4 * LAT;
-constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
 
 struct LowpArgs {
     const uint32_t *ahi, *alo;   // [NATOMS][ld] bf16x2
@@ -495,13 +596,6 @@ __device__ __forceinline__ uint32_t swap_neg(uint32_t w)
 __device__ __forceinline__ u32x4 swap_neg4(u32x4 x)
 {
     return u32x4{swap_neg(x[0]), swap_neg(x[1]), swap_neg(x[2]), swap_neg(x[3])};
-}
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const uint32_t *p, size_t bytes)
-{
-    const uint32_t n = bytes > 0xffffffffu ? 0xffffffffu : (uint32_t)bytes;
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(p), (short)0, (int)n,
-                                             (int)kRsrcWord3);
 }
 
 // 16 bytes at byte offset voff + soff; zeros when voff is past the resource
@@ -1240,7 +1334,13 @@ void launch_fp32(const MatchLaunch &l, const Scratch &sc, const float2 *atoms, s
     a.chunk = chunk;
     const size_t chunks = (nat + chunk - 1) / chunk;
     dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
-    match_fp32_kernel<<<grid, kFp32Threads, 0, s>>>(a);
+    // direct-to-LDS copies need 16-byte rows (even T) and byte offsets into
+    // the signals below kOob
+    a.sig_bytes = l.V * (size_t)l.T * sizeof(float2);
+    if ((l.T & 1) == 0 && a.sig_bytes < kOob && (size_t)AT * l.T * sizeof(float2) < kOob)
+        match_fp32_kernel<true><<<grid, kFp32Threads, 0, s>>>(a);
+    else
+        match_fp32_kernel<false><<<grid, kFp32Threads, 0, s>>>(a);
 }
 
 // tile-ordered BF16 (and lo) copies of the signals
