@@ -95,7 +95,9 @@ __device__ __forceinline__ float4 load_pair(const float2 *row, int t, int T, boo
 constexpr int kFp32AtomGroups = 4;
 constexpr int kFp32Threads = 128 * kFp32AtomGroups;
 constexpr uint32_t kRsrcWord3 = 0x00020000u;   // raw 32-bit buffer, gfx9 family
-constexpr uint32_t kOob = 0x80000000u;         // a buffer offset past every resource used here
+// a buffer offset past every resource used here (resources stay at or below
+// it, and offset + 16 does not wrap)
+constexpr uint32_t kOob = 0xFFFFFF00u;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, size_t bytes)
 {
@@ -106,7 +108,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, size_
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// DMA = true (even T, signals below 2 GB): the chunks go from global memory
+// DMA = true (even T, signals below 4 GB): the chunks go from global memory
 // straight into LDS (buffer_load ... lds), no registers or LDS stores on the
 // way.  A wave's copy of 64 x 16 bytes lands as one 1 KB block, lane l at
 // byte 16 l, so the LDS holds a chunk fragment-major: block (operand, row
@@ -1334,10 +1336,10 @@ void launch_fp32(const MatchLaunch &l, const Scratch &sc, const float2 *atoms, s
     a.chunk = chunk;
     const size_t chunks = (nat + chunk - 1) / chunk;
     dim3 grid((unsigned)sc.max_tiles, (unsigned)chunks);
-    // direct-to-LDS copies need 16-byte rows (even T) and byte offsets into
-    // the signals below kOob
+    // direct-to-LDS copies need 16-byte rows (even T) and 32-bit byte offsets
+    // into the signals and a sub-tile's atoms, below kOob
     a.sig_bytes = l.V * (size_t)l.T * sizeof(float2);
-    if ((l.T & 1) == 0 && a.sig_bytes < kOob && (size_t)AT * l.T * sizeof(float2) < kOob)
+    if ((l.T & 1) == 0 && a.sig_bytes <= kOob && (size_t)AT * l.T * sizeof(float2) <= kOob)
         match_fp32_kernel<true><<<grid, kFp32Threads, 0, s>>>(a);
     else
         match_fp32_kernel<false><<<grid, kFp32Threads, 0, s>>>(a);
