@@ -471,8 +471,9 @@ def differentiation(sm, torch, np, raw, T, w, d_atoms, d_params, params, A, nato
         "adjoint": {"ms": adj_ms, "atoms_per_s": V / (adj_ms * 1e-3),
                     "tflops_algorithmic": af / (adj_ms * 1e-3) / 1e12,
                     "frac_fp32_peak": af / (adj_ms * 1e-3) / 1e12 / PEAK_FP32,
-                    "flop_per_state_tr": FLOP_ADJ, "checkpoint_every": 5,
-                    "segment_states": "LDS",
+                    "flop_per_state_tr": FLOP_ADJ, "checkpoint_every": 4,
+                    "lanes_per_atom": 8 if w <= 32 else 16,
+                    "segment_states": "first and last in registers, middle two in LDS",
                     "traffic_per_call": measured_traffic("adjoint"),
                     "valu_issue": measured_counters("adjoint", "valu")},
         "gauss_newton": {"ms": gn_ms, "iters": iters, "voxels_per_s": V / (gn_ms * 1e-3),
@@ -840,7 +841,7 @@ def main() -> int:
         out["roofline_match"] = {"bound": "mfma", "achieved": ach_m, "peak": PEAK_FP32,
                                  "unit": "TFLOP/s", "frac": ach_m / PEAK_FP32,
                                  "traffic": measured_traffic("match"),
-                                 "kernel": "match_fp32_kernel",
+                                 "kernel": "match_fp32_kernel<true>",
                                  "algorithmic_flops_per_launch": mflops,
                                  # three real products per complex product:
                                  # the MFMA pipes issue 6 of the 8 flops
