@@ -624,8 +624,8 @@ __device__ __forceinline__ Atom load_atom(const float *theta, size_t a)
 }
 
 // ---------------------------------------------------------------------------
-// Jacobian: s [n][T] and J [n][4][T] (either may be null)
-// atom of the calling lane's group and the lane's state index
+// Lane-per-state passes (the CRB gradient and the Gauss-Newton primal pass):
+// atom of the calling lane's group
 template <int SUB>
 __device__ __forceinline__ size_t group_atom()
 {
@@ -633,43 +633,6 @@ __device__ __forceinline__ size_t group_atom()
            ((threadIdx.x & 63) / SUB);
 }
 
-template <int SUB>
-__global__ __launch_bounds__(64 * kWaves) void jac_kernel(const DTr *seq, int T, int w,
-                                                          const float *theta, size_t n,
-                                                          float2 *sig, float2 *jac)
-{
-    const size_t a = group_atom<SUB>();
-    if (a >= n) return;
-    const int k = threadIdx.x & (SUB - 1);
-    const Atom p = load_atom(theta, a);
-    __shared__ TrA coef_lds[kCoefLds];
-    __shared__ float2 read_lds[kReadLds];
-    CoefCache<SUB> cc = coef_cache<SUB>(coef_lds);
-    float2 *rd = read_stage<SUB>(read_lds);
-    Aug<4> y;
-    f_init(y, p, k, frame0(seq));
-    for (int t = 0; t < T; t++) {
-        const TrK c = coef<SUB>(cc, seq, T, p, t, k);
-        Aug<4> r;
-        f_tr<4, SUB>(y, c, p, k, w, &r);
-        const int j = t % SUB;
-        if (k == 0)   // readouts in true coordinates
-#pragma unroll
-            for (int d = 0; d < 5; d++) rd[j * 5 + d] = c_mul(c.ro, r.fp[d]);
-        if (j == SUB - 1 || t == T - 1) {
-            // lane k stores TR t - j + k: coalesced rows of s and J
-            wave_sync();
-            if (k <= j) {
-                const size_t tt = (size_t)(t - j + k);
-                if (sig) sig[a * T + tt] = rd[k * 5];
-                if (jac)
-#pragma unroll
-                    for (int d = 0; d < 4; d++) jac[(a * 4 + d) * T + tt] = rd[k * 5 + 1 + d];
-            }
-            wave_sync();
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Reverse sweeps.  ND = 0: L = Re sum conj(v_n) s_n (adjoint).  ND = 4: the
@@ -1395,6 +1358,196 @@ __global__ __launch_bounds__(64, 2) void adj_kernel(RevArgs A)
 }
 
 // ---------------------------------------------------------------------------
+// Forward mode in the lane-group layout (the Jacobian and the Gauss-Newton
+// tangent passes): the adjoint's layout (L lanes per atom, lane j holding
+// states 4j .. 4j + 3) with each state's four tangents (T1, T2, B0, B1), 120
+// registers per lane.  The tangent rules are those of f_rf / f_relax /
+// f_phase, rearranged like the adjoint's forward steps: relax(TE) and
+// phase(TE) are one factor F = e2 e^{i beta TE} on F+ (conj(F) on F-), and
+// so are relax(TR - TE) and phase(TR - TE) after the shift, with
+// their T2 and B0 source terms folded in before it as y_T2 += (t / T2^2) y
+// and y_B0 += i c t y (on F-: -i c t y), since d(e2)/dT2 = e2 t / T2^2 and
+// d(e^{i beta t})/dB0 = i c t e^{i beta t}; the truncation is the adjoint's
+// (states >= w stay zero, F+_w <- 0 in the relaxation after the shift).
+constexpr int kJd = 5;   // the state and its four tangents
+
+struct JState {
+    float2 fp[kAdjM][kJd], fm[kAdjM][kJd], z[kAdjM][kJd];
+};
+
+// init_rf_pulse with the tangents of its T1/T2 dependence (f_init)
+__device__ __forceinline__ void j_init(JState &y, const Atom &p, const AdjLane &ln, float2 ro0)
+{
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++)
+#pragma unroll
+        for (int d = 0; d < kJd; d++) y.fp[i][d] = y.fm[i][d] = y.z[i][d] = {0.f, 0.f};
+    if (!ln.first) return;
+    const float a0 = (float)(180.0 * M_PI / 180.0f);
+    const float sa0 = sinf(a0), ca0 = cosf(a0);
+    const float e1 = expf(-kTI * p.iT1), e2 = expf(-kTI * p.iT2);
+    const float2 tmp = {0.f, e2 * sa0};
+    y.fp[0][0] = c_mul(c_conj(ro0), c_conj(tmp));
+    y.fm[0][0] = c_mul(ro0, tmp);
+    y.z[0][0] = {e1 * ca0 + 1.f - e1, 0.f};
+    const float e1d = e1 * kTI * p.iT1 * p.iT1, e2d = e2 * kTI * p.iT2 * p.iT2;
+    y.z[0][1] = {e1d * (ca0 - 1.f), 0.f};
+    const float2 t2 = {0.f, e2d * sa0};
+    y.fp[0][2] = c_mul(c_conj(ro0), c_conj(t2));
+    y.fm[0][2] = c_mul(ro0, t2);
+}
+
+// RF: every component through M(alpha); the B1 tangent also gets alpha M_a y
+__device__ __forceinline__ void j_rf(JState &y, const AdjCoef &c)
+{
+    const C4 m = rf_c(c.sa, c.ca), m1 = rf_d1(c.sa, c.ca);
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        float2 sp, sm, sz;
+        rf3<false>(m1, y.fp[i][0], y.fm[i][0], y.z[i][0], sp, sm, sz);
+#pragma unroll
+        for (int d = 0; d < kJd; d++)
+            rf3<true>(m, y.fp[i][d], y.fm[i][d], y.z[i][d], y.fp[i][d], y.fm[i][d], y.z[i][d]);
+        y.fp[i][4] = c_add(y.fp[i][4], c_scale(sp, c.al));
+        y.fm[i][4] = c_add(y.fm[i][4], c_scale(sm, c.al));
+        y.z[i][4] = c_add(y.z[i][4], c_scale(sz, c.al));
+    }
+}
+
+// B0 source of a precession over c t (before its factor)
+__device__ __forceinline__ void j_b0_source(JState &y, float ct)
+{
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        y.fp[i][3] = c_add(y.fp[i][3], c_scale(c_i(y.fp[i][0]), ct));
+        y.fm[i][3] = c_add(y.fm[i][3], c_scale(c_i(y.fm[i][0]), -ct));
+    }
+}
+
+// relax over an interval t with the precession over it: F (e2 times the
+// precession factor) on F+, conj(F) on F-, e1 on Z with the source on state
+// 0; g2 = t / T2^2 and e1d = e1 t / T1^2 give the T2 and T1 source terms.
+// MASK: F+_w <- 0.
+template <bool MASK>
+__device__ __forceinline__ void j_decay(JState &y, float2 F, float e1, float g2, float e1d,
+                                        const AdjLane &ln)
+{
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        float2 zr = y.z[i][0];
+        if (i == 0 && ln.first) zr.x -= 1.f;
+        y.fp[i][2] = c_add(y.fp[i][2], c_scale(y.fp[i][0], g2));
+        y.fm[i][2] = c_add(y.fm[i][2], c_scale(y.fm[i][0], g2));
+        const float2 Fi = (MASK && ln.atw[i]) ? float2{0.f, 0.f} : F;
+        const float2 Qi = c_conj(F);
+#pragma unroll
+        for (int d = 0; d < kJd; d++) {
+            y.fp[i][d] = c_mul(Fi, y.fp[i][d]);
+            y.fm[i][d] = c_mul(Qi, y.fm[i][d]);
+        }
+#pragma unroll
+        for (int d = 0; d < kJd; d++) y.z[i][d] = c_scale(y.z[i][d], e1);
+        y.z[i][1] = c_add(y.z[i][1], c_scale(zr, e1d));
+    }
+    if (ln.first) y.z[0][0].x += 1.f - e1;
+}
+
+// the shift of every component (a_shift)
+__device__ __forceinline__ void j_shift(JState &y, const AdjLane &ln)
+{
+#pragma unroll
+    for (int d = 0; d < kJd; d++) {
+        const float2 nxt = row_shl1(y.fm[0][d]);
+        const float2 prv = row_shr1(y.fp[kAdjM - 1][d]);
+#pragma unroll
+        for (int i = 0; i < kAdjM - 1; i++) y.fm[i][d] = y.fm[i + 1][d];
+        y.fm[kAdjM - 1][d] = ln.last ? float2{0.f, 0.f} : nxt;
+#pragma unroll
+        for (int i = kAdjM - 1; i > 0; i--) y.fp[i][d] = y.fp[i - 1][d];
+        y.fp[0][d] = ln.first ? c_conj(y.fm[0][d]) : prv;
+    }
+}
+
+// one TR; the readouts (state 0 and its tangents, true coordinates) into rd
+// on the group's first lane
+__device__ __forceinline__ void j_tr(JState &y, const AdjCoef &c, const Atom &p,
+                                     const AdjLane &ln, float2 *rd)
+{
+    const float i1s = p.iT1 * p.iT1, i2s = p.iT2 * p.iT2;
+    j_rf(y, c);
+    j_b0_source(y, kCB0 * c.te);
+    j_decay<false>(y, c_scale(c.pa, c.e2a), c.e1a, c.te * i2s, c.e1a * c.te * i1s, ln);
+    if (ln.first)
+#pragma unroll
+        for (int d = 0; d < kJd; d++) rd[d] = c_mul(c.ro, y.fp[0][d]);
+    // phase(TR - TE) commutes with the shift (a uniform factor on F+, its
+    // conjugate on F-, and conj(conj(P) F-) = P conj(F-)), so it joins the
+    // relaxation after the shift, its B0 source with it
+    j_shift(y, ln);
+    j_b0_source(y, kCB0 * c.t2);
+    j_decay<true>(y, c_scale(c.pb, c.e2b), c.e1b, c.t2 * i2s, c.e1b * c.t2 * i1s, ln);
+}
+
+// the lane's place in its group of L (all lanes of a wave, 64 / L atoms)
+template <int L>
+__device__ __forceinline__ AdjLane adj_lane(int lane, int w)
+{
+    AdjLane ln;
+    ln.j = lane & (L - 1);
+    ln.first = ln.j == 0;
+    ln.last = ln.j == L - 1;
+#pragma unroll
+    for (int i = 0; i < kAdjM; i++) {
+        ln.atw[i] = kAdjM * ln.j + i == w;
+        ln.atw1[i] = kAdjM * ln.j + i == w - 1;
+    }
+    return ln;
+}
+
+// One atom's forward sweep with tangents; every L TRs (and at the end) the
+// group's lanes take one TR each of the staged readouts: fold(t, e) with e
+// the state and its four tangents at TR t.
+template <int L, typename Fold>
+__device__ __forceinline__ void j_sweep(const DTr *seq, int T, const Atom &p, const AdjLane &ln,
+                                        AdjCoef *coef, float2 (*rd)[kJd], Fold fold)
+{
+    AdjCache<L> cc{coef, -1};
+    JState y;
+    j_init(y, p, ln, frame0(seq));
+    for (int t = 0; t < T; t++) {
+        const int jj = t % L;
+        j_tr(y, adj_coef<L>(cc, seq, T, p, t, ln.j), p, ln, rd[jj]);
+        if (jj == L - 1 || t == T - 1) {
+            wave_sync();
+            if (ln.j <= jj) fold(t - jj + ln.j, rd[ln.j]);
+            wave_sync();
+        }
+    }
+}
+
+// Jacobian: s [n][T] and J [n][4][T] (either may be null); one wave per
+// work-group, two per SIMD
+template <int L>
+__global__ __launch_bounds__(64, 2) void jacl_kernel(const DTr *seq, int T, int w,
+                                                     const float *theta, size_t n, float2 *sig,
+                                                     float2 *jac)
+{
+    __shared__ AdjCoef coef_lds[64];
+    __shared__ float2 rd_lds[64 / L][L][kJd];
+    const int lane = threadIdx.x;
+    const size_t a = (size_t)blockIdx.x * (64 / L) + lane / L;
+    if (a >= n) return;   // whole groups: their lanes only meet their own group
+    const AdjLane ln = adj_lane<L>(lane, w);
+    j_sweep<L>(seq, T, load_atom(theta, a), ln, coef_lds + (lane & ~(L - 1)), rd_lds[lane / L],
+               [&](int t, const float2 *e) {
+                   if (sig) sig[a * T + t] = e[0];
+                   if (jac)
+#pragma unroll
+                       for (int d = 0; d < 4; d++) jac[(a * 4 + d) * T + t] = e[1 + d];
+               });
+}
+
+// ---------------------------------------------------------------------------
 // Gauss-Newton (Levenberg-Marquardt damped) refinement of (T1, T2, B0, B1)
 // per voxel with the complex proton density eliminated (variable
 // projection): cost c = |x|^2 - |s^H x|^2 / |s|^2.  A step is kept only if
@@ -1475,6 +1628,39 @@ __device__ GnSums gn_pass(const DTr *seq, int T, int w, const Atom &p, const flo
     return g;
 }
 
+// gn_pass with tangents in the lane-group layout: the group's lanes fold
+// one TR each of every L staged readouts
+template <int L>
+__device__ GnSums gnl_pass(const DTr *seq, int T, const Atom &p, const float2 *x, const AdjLane &ln,
+                           AdjCoef *coef, float2 (*rd)[kJd])
+{
+    GnSums g{};
+    j_sweep<L>(seq, T, p, ln, coef, rd, [&](int t, const float2 *e) {
+        const float2 s = e[0], xv = x[t];
+        g.S += re_dot(s, s);
+        g.Sx = c_add(g.Sx, c_mul(c_conj(s), xv));
+        int i = 0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const float2 ds = e[1 + d];
+            g.U[d] = c_add(g.U[d], c_mul(c_conj(s), ds));
+            g.W[d] = c_add(g.W[d], c_mul(c_conj(ds), xv));
+#pragma unroll
+            for (int q = d; q < 4; q++, i++) g.D[i] = c_add(g.D[i], c_mul(c_conj(ds), e[1 + q]));
+        }
+    });
+    g.S = adj_gsum<L>(g.S);
+    g.Sx = {adj_gsum<L>(g.Sx.x), adj_gsum<L>(g.Sx.y)};
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        g.U[d] = {adj_gsum<L>(g.U[d].x), adj_gsum<L>(g.U[d].y)};
+        g.W[d] = {adj_gsum<L>(g.W[d].x), adj_gsum<L>(g.W[d].y)};
+    }
+#pragma unroll
+    for (int i = 0; i < 10; i++) g.D[i] = {adj_gsum<L>(g.D[i].x), adj_gsum<L>(g.D[i].y)};
+    return g;
+}
+
 __device__ __forceinline__ bool theta_ok(const float *th)
 {
     return th[0] > 0.f && th[1] > 0.f && th[3] > 0.f && isfinite(th[0]) && isfinite(th[1]) &&
@@ -1513,19 +1699,22 @@ static_assert(offsetof(GnState, trial) == kGnRecTrial * sizeof(float) &&
                   sizeof(GnState) % sizeof(float) == 0,
               "Gauss-Newton record layout");
 
-// group's voxel: one group of SUB lanes per voxel (or per list entry)
-template <int SUB>
-__global__ __launch_bounds__(64 * kWaves) void gn_init_kernel(GnArgs A)
+// group's voxel: one group of L lanes per voxel (or per list entry), one
+// wave per work-group
+template <int L>
+__global__ __launch_bounds__(64, 2) void gn_init_kernel(GnArgs A)
 {
-    const size_t v = group_atom<SUB>();
-    if (v >= A.V) return;
-    __shared__ TrA coef_lds[kCoefLds];
-    __shared__ float2 read_lds[kReadLds];
-    const int k = threadIdx.x & (SUB - 1);
+    __shared__ AdjCoef coef_lds[64];
+    __shared__ float2 rd_lds[64 / L][L][kJd];
+    const int lane = threadIdx.x;
+    const size_t v = (size_t)blockIdx.x * (64 / L) + lane / L;
+    if (v >= A.V) return;   // whole groups: their lanes only meet their own group
+    const AdjLane ln = adj_lane<L>(lane, A.w);
+    const int k = ln.j;
     const float2 *x = A.sig + v * A.T;
     float xx = 0.f;
-    for (int t = k; t < A.T; t += SUB) xx += re_dot(x[t], x[t]);
-    xx = group_sum<SUB>(xx);
+    for (int t = k; t < A.T; t += L) xx += re_dot(x[t], x[t]);
+    xx = adj_gsum<L>(xx);
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = A.theta[v * 4 + d];
@@ -1537,7 +1726,8 @@ __global__ __launch_bounds__(64 * kWaves) void gn_init_kernel(GnArgs A)
     if (ok)
 #pragma unroll
         for (int d = 0; d < 4; d++) tp[d] = th[d];
-    GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(tp), x, k, coef_lds, read_lds);
+    GnSums g = gnl_pass<L>(A.seq, A.T, make_atom(tp), x, ln, coef_lds + (lane & ~(L - 1)),
+                           rd_lds[lane / L]);
     if (k == 0) {
         GnState &st = A.st[v];
         float c = xx - (g.Sx.x * g.Sx.x + g.Sx.y * g.Sx.y) / g.S;
@@ -1651,22 +1841,23 @@ __global__ void gn_accept_kernel(GnArgs A, int it)
 }
 
 // tangent pass at the new estimate of the listed voxels
-template <int SUB>
-__global__ __launch_bounds__(64 * kWaves) void gn_tangent_kernel(GnArgs A)
+template <int L>
+__global__ __launch_bounds__(64, 2) void gn_tangent_kernel(GnArgs A)
 {
-    const size_t p = group_atom<SUB>();
+    __shared__ AdjCoef coef_lds[64];
+    __shared__ float2 rd_lds[64 / L][L][kJd];
+    const int lane = threadIdx.x;
+    const size_t p = (size_t)blockIdx.x * (64 / L) + lane / L;
     if (p >= (size_t)*A.count) return;
-    __shared__ TrA coef_lds[kCoefLds];
-    __shared__ float2 read_lds[kReadLds];
-    const int k = threadIdx.x & (SUB - 1);
+    const AdjLane ln = adj_lane<L>(lane, A.w);
     const size_t v = (size_t)A.list[p];
     GnState &st = A.st[v];
     float th[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) th[d] = st.th[d];
-    const GnSums g = gn_pass<4, SUB>(A.seq, A.T, A.w, make_atom(th), A.sig + v * A.T, k, coef_lds,
-                                     read_lds);
-    if (k == 0) st.g = g;
+    const GnSums g = gnl_pass<L>(A.seq, A.T, make_atom(th), A.sig + v * A.T, ln,
+                                 coef_lds + (lane & ~(L - 1)), rd_lds[lane / L]);
+    if (ln.first) st.g = g;
 }
 
 __global__ void gn_final_kernel(GnArgs A)
@@ -1704,9 +1895,9 @@ int launch_jacobian(const DTr *seq, int T, int w, const float *theta, size_t n, 
 {
     if (n == 0) return 0;
     if (w <= 32)
-        jac_kernel<32><<<groups(n, 32), 64 * kWaves, 0, s>>>(seq, T, w, theta, n, sig, jac);
+        jacl_kernel<8><<<(unsigned)((n + 7) / 8), 64, 0, s>>>(seq, T, w, theta, n, sig, jac);
     else
-        jac_kernel<64><<<groups(n, 64), 64 * kWaves, 0, s>>>(seq, T, w, theta, n, sig, jac);
+        jacl_kernel<16><<<(unsigned)((n + 3) / 4), 64, 0, s>>>(seq, T, w, theta, n, sig, jac);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : mrf_fail(MRF_E_GPU, "jacobian launch: %s", hipGetErrorString(e));
 }
@@ -1796,8 +1987,10 @@ int launch_gn(const DTr *seq, const TrCoef *coef, int T, int w, const float2 *si
     const int sub = w <= 32 ? 32 : 64;
     const dim3 gg = groups(V, sub), gb(64 * kWaves);
     const dim3 vb((unsigned)((V + 255) / 256));
-    if (sub == 32) gn_init_kernel<32><<<gg, gb, 0, s>>>(A);
-    else gn_init_kernel<64><<<gg, gb, 0, s>>>(A);
+    // the tangent passes: L = 8 (16) lanes per voxel, one wave per work-group
+    const dim3 lg((unsigned)(w <= 32 ? (V + 7) / 8 : (V + 3) / 4));
+    if (w <= 32) gn_init_kernel<8><<<lg, 64, 0, s>>>(A);
+    else gn_init_kernel<16><<<lg, 64, 0, s>>>(A);
     for (int it = 0; it < iters; it++) {
         gn_solve_kernel<<<vb, 256, 0, s>>>(A);
         // the trial cost: one voxel per lane where nstates allows, else
@@ -1809,8 +2002,8 @@ int launch_gn(const DTr *seq, const TrCoef *coef, int T, int w, const float2 *si
         }
         gn_accept_kernel<<<vb, 256, 0, s>>>(A, it);
         if (it + 1 < iters) {
-            if (sub == 32) gn_tangent_kernel<32><<<gg, gb, 0, s>>>(A);
-            else gn_tangent_kernel<64><<<gg, gb, 0, s>>>(A);
+            if (w <= 32) gn_tangent_kernel<8><<<lg, 64, 0, s>>>(A);
+            else gn_tangent_kernel<16><<<lg, 64, 0, s>>>(A);
         }
     }
     gn_final_kernel<<<vb, 256, 0, s>>>(A);

@@ -126,7 +126,10 @@ def test_adjoint_many_atoms(gpu, w):
 
 
 def test_adjoint_dot_product_on_gpu(gpu):
-    """<J u, v> = <u, J^T v> between the GPU's own forward and reverse modes."""
+    """<J u, v> = <u, J^T v> between the GPU's own forward and reverse modes.
+    With random cotangents a component can be a sum of cancelling terms (the
+    B1 component of atom 1 is 1.8e-3 of sum_t |v_t| |ds_t/dB1|), so the bound
+    adds 2e-6 of that unsigned sum (1e-6 per side) to the 1e-4 relative."""
     T, w = 300, 24
     raw = _raw(T, seed=9)
     n = len(THETAS)
@@ -140,7 +143,8 @@ def test_adjoint_dot_product_on_gpu(gpu):
     g_th = g_th.cpu().numpy()
     for a in range(n):
         want = np.array([np.real(np.vdot(v[a].astype(np.complex128), J[a, d])) for d in range(4)])
-        assert np.all(np.abs(g_th[a] - want) <= TOL * np.abs(want).max()), a
+        scale = np.abs(J[a]).astype(np.float64) @ np.abs(v[a]).astype(np.float64)
+        assert np.all(np.abs(g_th[a] - want) <= TOL * np.abs(want).max() + 2e-6 * scale), a
 
 
 def test_adjoint_flip_angle_central_differences(gpu):
