@@ -25,7 +25,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsnapmrf.so")
 CLI = os.path.join(HERE, "bin", "mrf")
 
-HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "grad.hip", "svd.hip", "capi.hip"]
+HIP_SOURCES = ["epg.hip", "match.hip", "lowp.hip", "grad.hip", "svd.hip", "order.hip", "capi.hip"]
 HOST_SOURCES = ["host.cpp"]
 ARCH = "gfx950"
 # rocBLAS / rocSOLVER: the plain GEMMs and the eigensolver of the SVD basis

@@ -104,6 +104,12 @@ inline size_t lowp_ld(size_t T) { return (T + 31) / 32 * 32; }
 int launch_lowp_prepare(const float2 *src, const int32_t *perm, size_t rows, int T, int ld,
                         uint32_t *hi, uint32_t *lo, hipStream_t s);
 
+// Voxel order of the candidate rescore (order.hip): temporary bytes of a
+// radix sort of V (key, voxel) pairs, and the sort by the low `bits` bits.
+size_t voxel_sort_tmp_bytes(size_t V);
+int sort_voxels(void *tmp, size_t tmp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                const uint32_t *vox_in, uint32_t *vox_out, size_t V, int bits, hipStream_t s);
+
 // Differentiation form (grad.hip) --------------------------------------
 
 // per-TR table of the continuous EPG-SSFP model (unit transform in double).

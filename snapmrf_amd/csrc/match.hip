@@ -955,6 +955,7 @@ struct RescoreArgs {
     uint32_t *best_i;
     float *maps;
     int32_t *idx;
+    const uint32_t *order;      // voxel of wave i (or null: voxel i)
 };
 
 __device__ __forceinline__ size_t atom_row(const RescoreArgs &r, uint32_t gi)
@@ -967,6 +968,42 @@ __device__ __forceinline__ size_t atom_row(const RescoreArgs &r, uint32_t gi)
 // Acc: double (default; products of two floats are exact, so the pick is the
 // exact argmax among the candidates) or float (sums in FP32, the order below:
 // per lane over t = lane + 64 i, then a butterfly over the 64 lanes).
+// Per voxel (one wave): the key of the rescore order, the global index of
+// its best low-precision candidate (first index on ties; ~0 for none), and
+// the voxel itself.
+template <int CK>
+__global__ __launch_bounds__(kBlock) void top1_kernel(const float *part_s, const uint32_t *part_i,
+                                                      int nchunks, size_t V, uint32_t *key,
+                                                      uint32_t *vox_out)
+{
+    const size_t vox = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (vox >= V) return;
+    float bs = -1.0f;
+    uint32_t bi = 0xffffffffu;
+    for (int e = lane; e < nchunks * CK; e += 64) {
+        const size_t o = ((size_t)(e / CK) * V + vox) * CK + e % CK;
+        const float sc = part_s[o];
+        const uint32_t id = part_i[o];
+        if (sc >= 0.0f && cand_better(sc, id, bs, bi)) {
+            bs = sc;
+            bi = id;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float os = __shfl_xor(bs, off);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off);
+        if (os >= 0.0f && cand_better(os, oi, bs, bi)) {
+            bs = os;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        key[vox] = bi;
+        vox_out[vox] = (uint32_t)vox;
+    }
+}
+
 template <int K, int CK, typename Acc>
 __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(RescoreArgs ra)
 {
@@ -980,9 +1017,15 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
     const int T = ra.T;
     constexpr int PER = kMaxChunks * CK / 64;   // entries per lane
     static_assert(kMaxChunks * CK % 64 == 0, "candidate entries per lane");
-    const size_t vox = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    // XCD-aware numbering: work-group L runs on XCD L % 8 and takes a
+    // contiguous eighth of the (ordered) voxels, so the voxels that share an
+    // XCD's L2 at one time are neighbours in the order
+    unsigned L = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) L = (L & 7u) * (gridDim.x >> 3) + (L >> 3);
+    const size_t wi = (size_t)L * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (vox >= V) return;
+    if (wi >= V) return;
+    const size_t vox = ra.order ? (size_t)ra.order[wi] : wi;
     const int n = nchunks * CK;
     float s[PER];
     uint32_t id[PER];
@@ -1164,6 +1207,10 @@ struct Scratch {
     float *part_s;
     uint32_t *part_i;
     size_t chunk, nchunks, part_chunks;
+    // the rescore's voxel order: best-candidate keys and voxels, sorted
+    uint32_t *rkey, *rvox, *rkey_s, *rvox_s;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
     size_t bytes;
 };
 
@@ -1248,6 +1295,14 @@ Scratch carve(void *base, const MatchLaunch &l)
         p += align256(s.part_chunks * l.V * ck * 4);
         s.part_i = (uint32_t *)p;
         p += align256(s.part_chunks * l.V * ck * 4);
+        uint32_t **ord[4] = {&s.rkey, &s.rvox, &s.rkey_s, &s.rvox_s};
+        for (uint32_t **q : ord) {
+            *q = (uint32_t *)p;
+            p += align256(l.V * 4);
+        }
+        s.sort_tmp_bytes = voxel_sort_tmp_bytes(l.V);
+        s.sort_tmp = p;
+        p += align256(s.sort_tmp_bytes);
     }
     s.bytes = (size_t)(p - (uint8_t *)base);
     return s;
@@ -1418,10 +1473,20 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     // lists of 16-timepoint chunks took 421 ms.  SPLIT2: 16, 4-entry.
     // SPLIT3: 16, 2-entry (their fragments of 32-timepoint chunks do not fit
     // the registers of two waves per SIMD).
-    const dim3 rg((unsigned)((l.V + 3) / 4));
+    // rescore grid: whole groups of eight work-groups (the XCD numbering)
+    const dim3 rg((unsigned)(((l.V + 3) / 4 + 7) / 8 * 8));
     const dim3 blk(kLowpThreads);
+    // the rescore order: voxels sorted by their best candidate's index
+    int kbits = 1;
+    while (kbits < 32 && (l.natoms * (size_t)l.n_slices >> kbits) != 0) kbits++;
+    r.order = sc.rvox_s;
 #define MRF_RESCORE(K, CK)                                                  \
     do {                                                                    \
+        top1_kernel<CK><<<(unsigned)((l.V + 3) / 4), kBlock, 0, s>>>(       \
+            sc.part_s, sc.part_i, (int)nchunks, l.V, sc.rkey, sc.rvox);     \
+        if (int rc = sort_voxels(sc.sort_tmp, sc.sort_tmp_bytes, sc.rkey, sc.rkey_s, sc.rvox, \
+                                 sc.rvox_s, l.V, kbits, s))                 \
+            return rc;                                                      \
         if (l.rescore_fp32) rescore_kernel<K, CK, float><<<rg, kBlock, 0, s>>>(r);   \
         else rescore_kernel<K, CK, double><<<rg, kBlock, 0, s>>>(r);                 \
     } while (0)
