@@ -432,15 +432,14 @@ __device__ __forceinline__ void f_init(Aug<ND> &y, const Atom &p, int k, float2 
 
 __device__ __forceinline__ float2 frame0(const DTr *seq) { return {seq[0].cph, seq[0].sph}; }
 
-// --- reverse operators ----------------------------------------------------
+// --- reverse operators over the tangent-augmented state (CRB) -------------
 // l holds the adjoint of the operator's output on entry and of its input on
 // exit; x is the operator's input.  ga / gt collect the lane's share of
-// dL/d(alpha_n) and dL/dt (t the interval of the operator); gth (ND == 0)
-// the share of dL/dtheta.
+// dL/d(alpha_n) and dL/dt (t the interval of the operator).
 
 template <int ND>
 __device__ __forceinline__ void r_relax(Aug<ND> &l, const Aug<ND> &x, float e1, float e2, float t,
-                                        const Atom &p, bool k0, float *gt, float gth[4])
+                                        const Atom &p, bool k0, float *gt)
 {
     const float i1 = p.iT1, i2 = p.iT2;
     const float e1d = e1 * t * i1 * i1, e2d = e2 * t * i2 * i2;
@@ -460,10 +459,6 @@ __device__ __forceinline__ void r_relax(Aug<ND> &l, const Aug<ND> &x, float e1, 
                  re_dot(l.z[1], zr) * e1dt;
         }
         *gt += g;
-    }
-    if constexpr (ND == 0) {
-        gth[0] += re_dot(l.z[0], zr) * e1d;
-        gth[1] += (re_dot(l.fp[0], x.fp[0]) + re_dot(l.fm[0], x.fm[0])) * e2d;
     }
     float2 lp0 = c_scale(l.fp[0], e2), lm0 = c_scale(l.fm[0], e2), lz0 = c_scale(l.z[0], e1);
     if constexpr (ND > 0) {
@@ -500,7 +495,7 @@ __device__ __forceinline__ void r_dephase(Aug<ND> &l, int k, int w)
 
 template <int ND>
 __device__ __forceinline__ void r_phase(Aug<ND> &l, const Aug<ND> &x, float2 P, float t,
-                                        float beta, float *gt, float gth[4])
+                                        float beta, float *gt)
 {
     const float2 Q = c_conj(P);
     const float ct = kCB0 * t;
@@ -522,8 +517,6 @@ __device__ __forceinline__ void r_phase(Aug<ND> &l, const Aug<ND> &x, float2 P, 
         }
         *gt += g;
     }
-    if constexpr (ND == 0)
-        gth[2] += re_dot(l.fp[0], c_scale(c_i(Px0), ct)) + re_dot(l.fm[0], c_scale(c_i(Qx0), -ct));
     float2 lp0 = c_mul(Q, l.fp[0]), lm0 = c_mul(P, l.fm[0]);
     if constexpr (ND > 0) {
         // conj(i c t P) = -i c t Q ; conj(-i c t Q) = i c t P
@@ -541,7 +534,7 @@ __device__ __forceinline__ void r_phase(Aug<ND> &l, const Aug<ND> &x, float2 P, 
 
 template <int ND>
 __device__ __forceinline__ void r_rf(Aug<ND> &l, const Aug<ND> &x, const TrK &c, float B1,
-                                     float *ga, float gth[4])
+                                     float *ga)
 {
     const C4 m = rf_c(c.sa, c.ca), m1 = rf_d1(c.sa, c.ca);
     float g_a = 0.f, g_al = 0.f;
@@ -563,7 +556,6 @@ __device__ __forceinline__ void r_rf(Aug<ND> &l, const Aug<ND> &x, const TrK &c,
         rf3_adj<false>(m1, l.fp[4], l.fm[4], l.z[4], sp, sm, sz);
     }
     *ga += B1 * g_a + g_al;
-    if constexpr (ND == 0) gth[3] += c.al * g_a;
 #pragma unroll
     for (int j = 0; j <= ND; j++) {
         float2 a, b, d;
@@ -766,7 +758,6 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
     Aug<ND> l;
 #pragma unroll
     for (int j = 0; j <= ND; j++) l.fp[j] = l.fm[j] = l.z[j] = {0.f, 0.f};
-    float gth[4] = {0.f, 0.f, 0.f, 0.f};
     const float beta = kCB0 * p.B0;
     // the checkpoint of a segment is loaded while the segment after it (in
     // time) is reversed
@@ -803,11 +794,11 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
                 Aug<ND> x = x3;
                 f_phase(x, c.pb, c.t2);
                 f_dephase<ND, SUB>(x, k, w);
-                r_relax(l, x, c.e1b, c.e2b, c.t2, p, k == 0, &gt, gth);
+                r_relax(l, x, c.e1b, c.e2b, c.t2, p, k == 0, &gt);
             }
             r_dephase<ND, SUB>(l, k, w);
             // phase(TR - TE) and the readout: input = x3
-            r_phase(l, x3, c.pb, c.t2, beta, &gt, gth);
+            r_phase(l, x3, c.pb, c.t2, beta, &gt);
             if (k == 0) {
                 // dL/dJ_e = 2 sum_d G_de J_d
                 float2 J[4];
@@ -825,10 +816,10 @@ __device__ __forceinline__ void rev_atom(const RevArgs &A, size_t a, float *gs_l
                 }
             }
             // phase(TE): input = x2
-            r_phase(l, x2, c.pa, c.te, beta, nullptr, gth);
+            r_phase(l, x2, c.pa, c.te, beta, nullptr);
             // relax(TE): input = x1
-            r_relax(l, x1, c.e1a, c.e2a, c.te, p, k == 0, nullptr, gth);
-            r_rf(l, yn, c, p.B1, &ga, gth);
+            r_relax(l, x1, c.e1a, c.e2a, c.te, p, k == 0, nullptr);
+            r_rf(l, yn, c, p.B1, &ga);
             if (A.grad_seq) {
                 ga = group_sum<SUB>(ga);
                 gt = group_sum<SUB>(gt);
