@@ -236,10 +236,10 @@ def precision_form(sm, torch, np, synth, raw, T, w, stream, steps, timed, oracle
     log(f"precision form bf16_top32 streamed (generation included): {st_ms:.1f} ms")
     # the precision form's headline: the fastest one-product BF16 mode that
     # meets the >= 99.9 % agreement bar
-    kernels = {"bf16_top8": "match_lowp_kernel<1,32,3,2> + rescore_kernel<8,24>",
-               "bf16_top16": "match_lowp_kernel<1,32,3,2> + rescore_kernel<16,24>",
-               "bf16_top32": "match_lowp_kernel<1,32,6,1> + rescore_kernel<32,48>",
-               "bf16_top64": "match_lowp_kernel<1,32,3,2> + rescore_kernel<64,24>"}
+    kernels = {"bf16_top8": "match_bf16_dma_kernel<3> + rescore_kernel<8,24>",
+               "bf16_top16": "match_bf16_dma_kernel<3> + rescore_kernel<16,24>",
+               "bf16_top32": "match_bf16_dma_kernel<3> + rescore_kernel<32,24>",
+               "bf16_top64": "match_bf16_dma_kernel<3> + rescore_kernel<64,24>"}
     ok = [n for n in kernels if res[n]["argmax_agree_fp32"] >= 0.999]
     best = max(ok, key=lambda n: res[n]["voxels_per_s"]) if ok else "bf16_top64"
     res["headline_mode"] = best

@@ -159,10 +159,11 @@ int mrf_match(float *d_maps, const mrf_complex *h_img,
 #define MRF_MATCH_SPLIT3 2      /* split-BF16 3 products (hi*hi + lo*hi + hi*lo) + top-8 + rescore */
 #define MRF_MATCH_BF16_TOP16 3  /* BF16 MFMA + top-16 + rescore */
 #define MRF_MATCH_SPLIT2 4      /* dictionary split (a_hi + a_lo) * s_hi + top-8 + rescore */
-#define MRF_MATCH_BF16_TOP32 5  /* BF16 MFMA + 48 candidates per chunk + top-32 + rescore */
-#define MRF_MATCH_BF16_TOP64 6  /* BF16 MFMA + 64 candidates per chunk + top-64 + rescore: for
-                                   dense grids (B0 steps of 2 Hz), where the FP32 argmax can
-                                   rank past 32 in BF16 */
+#define MRF_MATCH_BF16_TOP32 5  /* BF16 MFMA + TOP16's candidates over twice its atom chunks
+                                   + top-32 + rescore */
+#define MRF_MATCH_BF16_TOP64 6  /* BF16 MFMA + candidates over 32 atom chunks + top-64 +
+                                   rescore: for dense grids (B0 steps of 2 Hz), where the FP32
+                                   argmax can rank past 32 in BF16 */
 /* Flag OR-ed into a low-precision mode: the candidates are rescored with FP32
  * sums instead of FP64 sums (FP64 makes the pick the exact argmax among the
  * candidates; FP32 keeps the rescore in the matcher's own precision). */

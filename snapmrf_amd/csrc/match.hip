@@ -648,6 +648,49 @@ __device__ __forceinline__ int row_atom(int s)
     return ((s >> 2) & 3) + 4 * (s & 3) + (s & ~15);
 }
 
+// Fold the |S|^2 of one sub-tile (a wave's NI x 4 blocks of Re and Im sums:
+// 16 NI atoms x 64 voxels) into the lane's top-LK lists at cs / ci.  16
+// scores at a time are tested against the list threshold th[v]; the (rare)
+// insertions run in a short divergent loop in increasing atom order.  Lane
+// (g, r16) holds rows 4 g + r of each block: atoms lane_atom0 + 4 r + 16 i
+// (of the sub-tile's first nvalid) against voxel w 64 + 16 v + r16.
+template <int NI, int LK>
+__device__ __forceinline__ void fold_subtile(const floatx4 (&acc_re)[NI][4],
+                                             const floatx4 (&acc_im)[NI][4], float (&th)[4],
+                                             float *cs, uint32_t *ci, int ah, int w, int g,
+                                             int r16, uint32_t lane_atom0, int nvalid)
+{
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int q = ((ah * 4 + g) * LVT + w * 64 + v * 16 + r16) * LK;
+#pragma unroll
+        for (int h = 0; h < NI / 4; h++) {
+            float sc[16];
+            uint32_t m = 0;
+#pragma unroll
+            for (int ii = 0; ii < 4; ii++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 4 * h + ii, j = 4 * ii + r;
+                    const float re = acc_re[i][v][r], im = acc_im[i][v][r];
+                    sc[j] = re * re + im * im;
+                    if (g + 4 * r + 16 * (ah * NI + i) < nvalid && sc[j] > th[v]) m |= 1u << j;
+                }
+            while (m) {
+                const int j = __builtin_ctz(m);
+                m &= m - 1;
+                float s1 = sc[0];
+#pragma unroll
+                for (int k = 1; k < 16; k++) s1 = (j == k) ? sc[k] : s1;
+                if (s1 > th[v])
+                    th[v] = lane_insert<LK>(&cs[q], &ci[q], s1,
+                                            lane_atom0 + 4u * (j & 3) +
+                                                16u * (ah * NI + 4 * h + (j >> 2)));
+            }
+        }
+    }
+}
+
 // KC timepoints per K chunk (16 or 32), NB LDS staging buffers (2: one
 // barrier per chunk; 1: a second barrier once the fragments are read, which
 // leaves LDS for longer lane lists); LK candidates per
@@ -873,45 +916,217 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
             }
             __syncthreads();
         }
-        // epilogue: fold |S|^2 of the sub-tile into this lane's top-4 lists.
-        // 16 scores at a time are tested against the list threshold; the
-        // (rare) insertions run in a short divergent loop in increasing atom
-        // order.
-        const uint32_t lane_atom0 = (uint32_t)(idx_base + sub) + (uint32_t)g;
-        const int nvalid = (int)nsub;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int q = ((ah * 4 + g) * LVT + w * 64 + v * 16 + r16) * LK;
-#pragma unroll
-            for (int h = 0; h < NI / 4; h++) {
-                float sc[16];
-                uint32_t m = 0;
-#pragma unroll
-                for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = 4 * h + ii, j = 4 * ii + r;
-                        const float re = acc_re[i][v][r], im = acc_im[i][v][r];
-                        sc[j] = re * re + im * im;
-                        if (g + 4 * r + 16 * (ah * NI + i) < nvalid && sc[j] > th[v]) m |= 1u << j;
-                    }
-                while (m) {
-                    const int j = __builtin_ctz(m);
-                    m &= m - 1;
-                    float s1 = sc[0];
-#pragma unroll
-                    for (int k = 1; k < 16; k++) s1 = (j == k) ? sc[k] : s1;
-                    if (s1 > th[v])
-                        th[v] = lane_insert<LK>(&cs[q], &ci[q], s1,
-                                            lane_atom0 + 4u * (j & 3) +
-                                                16u * (ah * NI + 4 * h + (j >> 2)));
-                }
-            }
-        }
+        fold_subtile<NI, LK>(acc_re, acc_im, th, cs, ci, ah, w, g, r16,
+                             (uint32_t)(idx_base + sub) + (uint32_t)g, (int)nsub);
         sub += a_step;
     }
     __syncthreads();
     // the 4 lane lists of each voxel are its 4 * LK candidates of this chunk
+    if (tid < tile.count) {
+        constexpr int CK = 4 * AH * LK;
+        const size_t vox = (size_t)a.perm[tile.pstart + tid];
+        float *ps = a.part_s + ((size_t)chunk_id * a.V + vox) * CK;
+        uint32_t *pi = a.part_i + ((size_t)chunk_id * a.V + vox) * CK;
+#pragma unroll
+        for (int gg = 0; gg < 4 * AH; gg++) {
+            const int q = (gg * LVT + tid) * LK;
+#pragma unroll
+            for (int j = 0; j < LK; j++) {
+                ps[gg * LK + j] = cs[q + j];
+                pi[gg * LK + j] = ci[q + j];
+            }
+        }
+    }
+}
+
+// One-product BF16 matcher with direct-to-LDS chunk copies (the TOP8,
+// TOP16 and TOP64 candidate kernel): the tile, waves, sub-tile order and
+// lane lists of match_lowp_kernel<1, 32, 3, 2>, with the 32-timepoint chunks
+// going from global memory straight into LDS (buffer_load ... lds) -- no
+// staging registers and no LDS stores in the MFMA stream.  A chunk row is
+// 128 bytes (32 bf16x2 words), eight 16-byte segments.  A copy (one
+// instruction of a wave) moves 8 rows: lane l takes row l / 8 of them,
+// segment (l mod 8) ^ key(row), and lands at byte 16 l of a 1 KB block, so
+// each copy reads whole cache lines and segment s of row r sits at 16-byte
+// slot 8 r + (s ^ key(r)), key(r) = (r + r / 8) mod 8 (r mod 16).  An MFMA
+// fragment (16 rows, lane group g reading segment 4 ks + g) then hits 16
+// different bank groups per 16 lanes.  The MFMA's K order within a chunk is
+// the segment order, the same for atoms and signals, so the sums are those
+// of the K-ordered product up to FP32 summation order.
+//
+// Per chunk c (two LDS buffers, one barrier): the second-half fragments of
+// c are read, the copies of chunk c + 1 go out (after the reads, so that no
+// read waits for them), the MFMAs of the first half and of half the second
+// run, the wave waits for its copies and meets the others at the barrier,
+// the first-half fragments of c + 1 are read, and the rest of c's MFMAs run
+// while they arrive.
+__device__ __forceinline__ int seg_key(int r)
+{
+    return (r + (r >> 3)) & 7;
+}
+
+template <int LK>
+__global__ __launch_bounds__(kLowpThreads) void match_bf16_dma_kernel(LowpArgs a)
+{
+    constexpr int AH = kAtomHalves;
+    constexpr int NT = kLowpThreads;
+    constexpr int NI = 8 / AH;              // 16-atom MFMA blocks per wave
+    constexpr int NW = NT / 64;             // waves
+    constexpr int CPA = LAT / 8 / NW;       // atom copies per wave and chunk
+    constexpr int CPX = LVT / 8 / NW;       // signal copies per wave and chunk
+    constexpr int BUF = (LAT + LVT) * 8;    // 16-byte slots per buffer: atom rows, then signal rows
+    static_assert(CPA * 8 * NW == LAT && CPX * 8 * NW == LVT, "copies cover the tile");
+    static_assert(16 * 2 * BUF + 4 * 2 * AH * 4 * LVT * LK <= 160 * 1024, "LDS budget");
+
+    // XCD-aware numbering as match_lowp_kernel: work-group L runs on XCD
+    // L % 8, and an XCD takes consecutive L.  Its concurrent work-groups
+    // (about 32, one per CU) take 4 voxel tiles x 8 atom chunks, so each atom
+    // sub-tile they stream is shared by four of them in the XCD's L2 while
+    // the four voxel tiles stay there; more than 8 chunks go 8 at a time.
+    unsigned L = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) L = (L & 7u) * (gridDim.x >> 3) + (L >> 3);
+    unsigned tile_u = L / (unsigned)a.nch, chunk_u = L % (unsigned)a.nch;
+    if (a.nch > 8 && (a.nch & 7) == 0) {
+        const unsigned ngrp = (unsigned)a.nch >> 3;   // chunk groups
+        const unsigned rest = L >> 5;
+        tile_u = (rest / ngrp) * 4 + ((L >> 3) & 3);
+        chunk_u = (rest % ngrp) * 8 + (L & 7);
+    }
+    const int tile_id = (int)tile_u;
+    const int chunk_id = (int)chunk_u;
+    if (tile_id >= *a.ntiles) return;
+    const Tile tile = a.tiles[tile_id];
+    const size_t a_end = a.natoms;
+    const size_t a_step = (size_t)a.nch * LAT;
+    const uint32_t ldb = (uint32_t)a.ld * 4u;   // row stride in bytes
+
+    // two chunk buffers
+    __shared__ u32x4 lb[2 * BUF];
+    __shared__ __attribute__((aligned(16))) float cs[AH * 4 * LVT * LK];
+    __shared__ __attribute__((aligned(16))) uint32_t ci[AH * 4 * LVT * LK];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: copy targets
+    const int w = wave & 3;                 // 64-voxel quarter of the tile
+    const int ah = wave >> 2;               // atom half
+    const int g = lane >> 4;
+    const int r16 = lane & 15;
+
+    for (int q = tid; q < AH * 4 * LVT * LK; q += NT) {
+        cs[q] = -1.0f;
+        ci[q] = 0xffffffffu;
+    }
+    float th[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+
+    const size_t xrow0 = (size_t)tile.pstart * a.ld;
+    const __amdgpu_buffer_rsrc_t rX = make_rsrc(a.shi + xrow0, (size_t)tile.count * ldb);
+    // the lane's byte offsets of its copies (row offset + segment); rows past
+    // the sub-tile or tile fall outside the resource and read as zeros
+    uint32_t aoff[CPA], xoff[CPX];
+#pragma unroll
+    for (int j = 0; j < CPA; j++) {
+        const int row = (wave * CPA + j) * 8 + (lane >> 3);
+        aoff[j] = (uint32_t)row_atom(row) * ldb + (uint32_t)((lane & 7) ^ seg_key(row & 15)) * 16u;
+    }
+#pragma unroll
+    for (int j = 0; j < CPX; j++) {
+        const int xr = (wave * CPX + j) * 8 + (lane >> 3);
+        xoff[j] = (uint32_t)xr * ldb + (uint32_t)((lane & 7) ^ seg_key(xr & 15)) * 16u;
+    }
+    // the lane's fragment slots within a 16-row block, per K half
+    const int fslot0 = r16 * 8 + ((0 + g) ^ seg_key(r16));
+    const int fslot1 = r16 * 8 + ((4 + g) ^ seg_key(r16));
+
+    int ks = 0;                             // slice of the launch
+    int r0 = chunk_id - a.phase;            // sub-tile residue of this chunk in slice ks
+    if (r0 < 0) r0 += a.nch;
+    size_t sub = (size_t)r0 * LAT;
+    while (true) {
+        if (sub >= a_end) {
+            if (++ks >= a.nsl) break;
+            r0 -= a.kstep;
+            if (r0 < 0) r0 += a.nch;
+            sub = (size_t)r0 * LAT;
+            continue;
+        }
+        const size_t row_base = (size_t)(tile.slice + ks) * a.row_stride;
+        const size_t idx_base = a.gidx_base + (size_t)(tile.slice + ks) * a.gidx_stride;
+        const size_t nsub = (a_end - sub < (size_t)LAT) ? a_end - sub : (size_t)LAT;
+        const __amdgpu_buffer_rsrc_t rA = make_rsrc(a.ahi + (row_base + sub) * a.ld, nsub * ldb);
+
+        floatx4 acc_re[NI][4], acc_im[NI][4];
+#pragma unroll
+        for (int i = 0; i < NI; i++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                acc_re[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc_im[i][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+        // chunk c's copies into buffer d (each wave: CPA atom and CPX signal
+        // copies of 8 rows)
+        auto copy_chunk = [&](int c, u32x4 *d) {
+            const int so = c * 128;
+#pragma unroll
+            for (int j = 0; j < CPA; j++)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rA, (lds_void *)&d[(wave * CPA + j) * 64], 16, (int)aoff[j], so, 0, 0);
+#pragma unroll
+            for (int j = 0; j < CPX; j++)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rX, (lds_void *)&d[LAT * 8 + (wave * CPX + j) * 64], 16, (int)xoff[j], so, 0, 0);
+        };
+        auto read_frags = [&](const u32x4 *d, int fslot, u32x4 (&fa)[NI], u32x4 (&fx)[4]) {
+#pragma unroll
+            for (int i = 0; i < NI; i++) fa[i] = d[((ah * NI + i) * 16) * 8 + fslot];
+#pragma unroll
+            for (int v = 0; v < 4; v++) fx[v] = d[(LAT + w * 64 + v * 16) * 8 + fslot];
+        };
+        auto mfmas = [&](const u32x4 (&fa)[NI], const u32x4 (&fx)[4], int i0, int i1) {
+            u32x4 fy[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) fy[v] = swap_neg4(fx[v]);
+#pragma unroll
+            for (int i = i0; i < i1; i++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    acc_re[i][v] = mfma_bf16(fa[i], fx[v], acc_re[i][v]);
+                    acc_im[i][v] = mfma_bf16(fa[i], fy[v], acc_im[i][v]);
+                }
+        };
+        u32x4 fa0[NI], fx0[4], fa1[NI], fx1[4];
+        const int nk = a.nk;
+        // (every read of both buffers ended before the previous sub-tile's
+        // last barrier)
+        copy_chunk(0, lb);
+        __builtin_amdgcn_s_waitcnt(0x0F70);     // vmcnt(0): this wave's copies landed
+        __syncthreads();
+        read_frags(lb, fslot0, fa0, fx0);
+        for (int c = 0; c < nk; c++) {
+            const bool more = c + 1 < nk;
+            u32x4 *cur = lb + (c & 1) * BUF;
+            u32x4 *nxt = lb + ((c + 1) & 1) * BUF;
+            read_frags(cur, fslot1, fa1, fx1);
+            __builtin_amdgcn_sched_barrier(0);
+            // the copies of chunk c + 1 go out after this chunk's reads (so no
+            // read waits for them); their buffer was last read in chunk c - 1,
+            // before the barrier that ended it
+            if (more) copy_chunk(c + 1, nxt);
+            mfmas(fa0, fx0, 0, NI);
+            mfmas(fa1, fx1, 0, NI / 2);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): chunk c + 1 landed
+            __syncthreads();
+            if (more) read_frags(nxt, fslot0, fa0, fx0);
+            __builtin_amdgcn_sched_barrier(0);
+            mfmas(fa1, fx1, NI / 2, NI);
+        }
+        fold_subtile<NI, LK>(acc_re, acc_im, th, cs, ci, ah, w, g, r16,
+                             (uint32_t)(idx_base + sub) + (uint32_t)g, (int)nsub);
+        sub += a_step;
+    }
+    __syncthreads();
+    // the lane lists of each voxel are its 4 AH LK candidates of this chunk
     if (tid < tile.count) {
         constexpr int CK = 4 * AH * LK;
         const size_t vox = (size_t)a.perm[tile.pstart + tid];
@@ -1222,7 +1437,7 @@ struct Scratch {
 int lane_k(int mode)
 {
     switch (mode) {
-    case MRF_MATCH_BF16_TOP32: return 6;
+    case MRF_MATCH_BF16_TOP32: return 3;
     case MRF_MATCH_BF16_TOP64: return 3;
     case MRF_MATCH_SPLIT2: return 4;
     case MRF_MATCH_SPLIT3: return 2;
@@ -1235,26 +1450,35 @@ int lane_k(int mode)
 // atom chunks per voxel tile: about 2048 work-groups, 8 to kMaxChunks.
 // BF16_TOP64 always takes kMaxChunks: its lane lists are the short 3-entry
 // ones of TOP8/TOP16, and lists that each cover 1/kMaxChunks of the atoms
-// keep the FP32 argmax among its 64 rescored candidates.
+// keep the FP32 argmax among its 64 rescored candidates.  BF16_TOP32 splits
+// each of TOP16's chunks in two where that stays within kMaxChunks (else it
+// takes TOP16's chunks): every TOP16 candidate is then a TOP32 candidate,
+// and at most 30 TOP32 candidates outscore any of TOP16's best 16 (an extra
+// candidate outscores it only where all 3 of its TOP16 list do, at most 5
+// lists), so TOP32 rescores every candidate TOP16 rescores.
 size_t lowp_want(size_t max_tiles, int mode)
 {
     if (mode == MRF_MATCH_BF16_TOP64) return kMaxChunks;
     size_t want = (2048 + max_tiles - 1) / max_tiles;
     if (want < 8) want = 8;
     if (want > (size_t)kMaxChunks) want = kMaxChunks;
+    if (mode == MRF_MATCH_BF16_TOP32 && 2 * want <= (size_t)kMaxChunks) want *= 2;
     return want;
 }
 
-// chunking of nat atoms; at most lowp_want chunks for any nat
+// chunking of nat atoms; at most lowp_want chunks for any nat (*chunk: the
+// atoms per chunk, for BF16_TOP32 those of the TOP16 chunk it splits)
 void lowp_chunks(size_t nat, size_t max_tiles, int mode, size_t *chunk, size_t *nchunks)
 {
-    const size_t want = lowp_want(max_tiles, mode);
+    const bool top32 = mode == MRF_MATCH_BF16_TOP32;
+    const size_t want = lowp_want(max_tiles, top32 ? MRF_MATCH_BF16_TOP16 : mode);
     size_t c = (nat + want - 1) / want;
     if (c < (size_t)kMinChunkAtoms) c = kMinChunkAtoms;
     c = (c + LAT - 1) / LAT * LAT;
     *chunk = c;
     *nchunks = (nat + c - 1) / c;
     if (*nchunks == 0) *nchunks = 1;
+    if (top32 && 2 * *nchunks <= (size_t)kMaxChunks) *nchunks *= 2;
 }
 
 Scratch carve(void *base, const MatchLaunch &l)
@@ -1455,24 +1679,24 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
     b.nsl = nsl;
     b.phase = (int)((a0 / LAT) % nchunks);
     b.kstep = (int)(nsub_w % nchunks);
-    const size_t nwg = (sc.max_tiles * nchunks + 7) / 8 * 8;
+    // whole groups of 4 tiles (the DMA kernel's numbering), whole groups of
+    // eight work-groups (the XCD numbering)
+    const size_t nwg = ((sc.max_tiles + 3) / 4 * 4 * nchunks + 7) / 8 * 8;
     if (nwg > 0x7fffffffULL) return mrf_fail(MRF_E_ARG, "too many work-groups");
     const dim3 grid((unsigned)nwg);
     r.nchunks = (int)nchunks;
     // Launch shapes measured on the 446k-atom x 460,800-voxel workload:
     // eight waves (two per SIMD) were 6-13 % faster than four waves of
-    // 128-atom tiles in every mode; the LDS (staging buffers + the lane
-    // lists) sets the chunk length and list size per mode.
-    // TOP8/TOP16: 32-timepoint chunks, 3-entry lists, two buffers.  TOP32:
-    // 6-entry lists (3-entry ones missed the exact pick on 0.11 % of voxels,
-    // 6-entry ones on 0.005 %) fit 32-timepoint chunks with one staging
-    // buffer (2 % faster than two buffers of 16).  TOP64: the TOP8/TOP16
-    // kernel over kMaxChunks strided chunks (32 x 24 candidates per voxel,
-    // top 64 rescored); on 500,000 atoms of the 871k-atom grid (B0 steps of
-    // 2 Hz) it agrees with FP32 on 99.92 % of voxels in 358 ms, where 8-entry
-    // lists of 16-timepoint chunks took 421 ms.  SPLIT2: 16, 4-entry.
-    // SPLIT3: 16, 2-entry (their fragments of 32-timepoint chunks do not fit
-    // the registers of two waves per SIMD).
+    // 128-atom tiles in every mode; the LDS (chunk buffers + the lane lists)
+    // sets the chunk length and list size per mode.
+    // TOP8/TOP16/TOP32/TOP64: match_bf16_dma_kernel (32-timepoint chunks
+    // copied straight to LDS, two buffers, 3-entry lists) over 8-32, 16 and
+    // 32 strided atom chunks; on 500,000 atoms of the 871k-atom grid (B0
+    // steps of 2 Hz) TOP32 keeps the FP32 pick on as many voxels as 6-entry
+    // lists over 8 chunks (99.66 %) in 282 ms against 359 ms, TOP64 on
+    // 99.92 % in 304 ms.  SPLIT2: match_lowp_kernel with 16-timepoint chunks
+    // and 4-entry lists.  SPLIT3: 16, 2-entry (their fragments of
+    // 32-timepoint chunks do not fit the registers of two waves per SIMD).
     // rescore grid: whole groups of eight work-groups (the XCD numbering)
     const dim3 rg((unsigned)(((l.V + 3) / 4 + 7) / 8 * 8));
     const dim3 blk(kLowpThreads);
@@ -1500,15 +1724,15 @@ int launch_lowp(const MatchLaunch &l, const Scratch &sc, const uint32_t *hi, con
         MRF_RESCORE(8, 32);
     } else if (l.mode == MRF_MATCH_BF16_TOP64) {
         b.nk = (l.T + 31) / 32;
-        match_lowp_kernel<1, 32, 3, 2><<<grid, blk, 0, s>>>(b);
+        match_bf16_dma_kernel<3><<<grid, blk, 0, s>>>(b);
         MRF_RESCORE(64, 24);
     } else if (l.mode == MRF_MATCH_BF16_TOP32) {
         b.nk = (l.T + 31) / 32;
-        match_lowp_kernel<1, 32, 6, 1><<<grid, blk, 0, s>>>(b);
-        MRF_RESCORE(32, 48);
+        match_bf16_dma_kernel<3><<<grid, blk, 0, s>>>(b);
+        MRF_RESCORE(32, 24);
     } else {
         b.nk = (l.T + 31) / 32;
-        match_lowp_kernel<1, 32, 3, 2><<<grid, blk, 0, s>>>(b);
+        match_bf16_dma_kernel<3><<<grid, blk, 0, s>>>(b);
         if (l.mode == MRF_MATCH_BF16_TOP16) MRF_RESCORE(16, 24);
         else MRF_RESCORE(8, 24);
     }

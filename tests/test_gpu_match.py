@@ -279,14 +279,15 @@ def test_lowp_b1_map_mode(gpu):
 
 
 def test_bf16_top32_candidates_contain_top16(gpu):
-    """BF16_TOP32 keeps 6 candidates per lane list (TOP16: 3) and rescores
-    the best 32 (TOP16: 16), so its candidate set contains TOP16's and its
-    FP64-rescored pick scores at least as high on every voxel; on a dense
+    """BF16_TOP32 runs TOP16's 3-entry lane lists over TOP16's atom chunks
+    split in two (here, at most 64 voxel tiles, TOP16 already takes the
+    maximum of 32 chunks and TOP32 takes the same ones) and rescores the best
+    32 (TOP16: 16), so every candidate TOP16 rescores is rescored by TOP32 and
+    its FP64-rescored pick scores at least as high on every voxel; on a dense
     grid (FP64 top-2 gaps below the BF16 rounding) it finds the FP64 argmax
-    at least as often as TOP16 and TOP8.  BF16_TOP64 runs TOP16's lane lists
-    (3 entries) over the same 32 atom chunks here (at most 64 voxel tiles)
-    and rescores the best 64 of them, so its candidate set contains TOP16's
-    and the same two properties hold against TOP16."""
+    at least as often as TOP16 and TOP8.  BF16_TOP64 runs the same lane lists
+    over the same 32 atom chunks and rescores the best 64 of them, so the
+    same two properties hold against TOP16."""
     T = 400
     raw = synth.csv_columns(synth.sequence(T, 4), "epg_ssfp")
     params, A, natoms = sm.build_params("300:10:900", "40:4:120", "-10:2:10", "1")
