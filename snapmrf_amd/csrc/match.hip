@@ -543,10 +543,14 @@ __global__ void zero_i32_kernel(int *p, size_t n)
 //
 // Work-group tile: 128 atoms x 256 voxels, 8 waves of 64 atoms x 64 voxels
 // (two per SIMD, one per atom half and voxel quarter; acc 4 x 4 MFMA blocks,
-// Re and Im); Y = (xi, -xr) is formed in registers from the X fragment.  K chunks of 16 or 32 timepoints are double-buffered
-// in LDS (one barrier per chunk).  Work-groups are numbered so that the ones on
-// one XCD (work-group id mod 8) take consecutive atom chunks of the same
-// voxel tile, which then streams from that XCD's L2.
+// Re and Im); Y = (xi, -xr) is formed in registers from the X fragment.
+// match_lowp_kernel (the split forms) stages K chunks of 16 or 32
+// timepoints through registers into unpadded LDS rows with a per-row
+// segment swizzle, double-buffered (one barrier per chunk);
+// match_bf16_dma_kernel (the one-product forms) copies 32-timepoint chunks
+// straight into LDS.  Work-groups are numbered so that the ones on one XCD
+// (work-group id mod 8) take consecutive atom chunks of the same voxel
+// tiles, which then stream from that XCD's L2.
 //
 // Candidates: MFMA row 4g + r of block i holds atom sub + g + 4r + 16i, so
 // lane group g sees the atoms = g (mod 4) and neighbouring atoms (adjacent
@@ -703,8 +707,13 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
     constexpr int NI = 8 / AH;              // 16-atom MFMA blocks per wave
     constexpr bool ASPLIT = NPROD >= 2;
     constexpr bool SSPLIT = NPROD >= 3;
-    constexpr int LSB = KC + 4;             // dwords per LDS row (padded)
+    constexpr int LSB = KC;                 // dwords per LDS row
     constexpr int SEGS = KC / 4;            // 16-byte segments per row chunk
+    // segment s of LDS row r sits at segment s ^ rkey(r) of the row: with
+    // these keys the fragment reads (16 rows, lane group g reading segment
+    // 4 ks + g) have no LDS bank conflicts (rows padded to KC + 4 words cost
+    // 4 conflict cycles per ds_read_b128, measured with SQ_LDS_BANK_CONFLICT)
+    auto rkey = [](int r) { return KC == 32 ? (r & 7) : (((r >> 2) & 1) << 1); };
     constexpr int RPP = NT / SEGS;          // rows staged per pass
     constexpr int PA = LAT / RPP;
     constexpr int PX = LVT / RPP;
@@ -813,13 +822,15 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
         auto stage_store = [&](int b) {
 #pragma unroll
             for (int p = 0; p < PA; p++) {
-                const int o = b * BA + (st_row + RPP * p) * LSB + (int)(st_b >> 2);
+                const int row = st_row + RPP * p;
+                const int o = b * BA + row * LSB + ((int)(st_b >> 4) ^ rkey(row)) * 4;
                 *reinterpret_cast<u32x4 *>(&lA[o]) = ga[p];
                 if (ASPLIT) *reinterpret_cast<u32x4 *>(&lAl[o]) = gal[p];
             }
 #pragma unroll
             for (int p = 0; p < PX; p++) {
-                const int o = b * BX + (st_row + RPP * p) * LSB + (int)(st_b >> 2);
+                const int row = st_row + RPP * p;
+                const int o = b * BX + row * LSB + ((int)(st_b >> 4) ^ rkey(row)) * 4;
                 *reinterpret_cast<u32x4 *>(&lX[o]) = gx[p];
                 if (SSPLIT) *reinterpret_cast<u32x4 *>(&lXl[o]) = gxl[p];
             }
@@ -831,7 +842,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
         auto read_frags = [&](int b) {
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                const int ko = ks * 16 + g * 4;
+                const int ko = ((ks * 4 + g) ^ rkey(r16)) * 4;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int o = b * BX + (w * 64 + v * 16 + r16) * LSB + ko;
@@ -946,13 +957,15 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 // staging registers and no LDS stores in the MFMA stream.  A chunk row is
 // 128 bytes (32 bf16x2 words), eight 16-byte segments.  A copy (one
 // instruction of a wave) moves 8 rows: lane l takes row l / 8 of them,
-// segment (l mod 8) ^ key(row), and lands at byte 16 l of a 1 KB block, so
-// each copy reads whole cache lines and segment s of row r sits at 16-byte
-// slot 8 r + (s ^ key(r)), key(r) = (r + r / 8) mod 8 (r mod 16).  An MFMA
-// fragment (16 rows, lane group g reading segment 4 ks + g) then hits 16
-// different bank groups per 16 lanes.  The MFMA's K order within a chunk is
-// the segment order, the same for atoms and signals, so the sums are those
-// of the K-ordered product up to FP32 summation order.
+// segment (l mod 8) ^ (row mod 8), and lands at byte 16 l of a 1 KB block,
+// so each copy reads whole cache lines and segment s of row r sits at
+// 16-byte slot 8 r + (s ^ (r mod 8)).  MFMA fragment reads (16 rows, lane
+// group g reading segment 4 ks + g) are then free of LDS bank conflicts
+// (measured with SQ_LDS_BANK_CONFLICT: a key of (r + r / 8) mod 8 costs 4
+// conflict cycles per ds_read_b128, rows padded to 36 words as many).  The
+// MFMA's K order within a chunk is the segment order, the same for atoms
+// and signals, so the sums are those of the K-ordered product up to FP32
+// summation order.
 //
 // Per chunk c (two LDS buffers, one barrier): the second-half fragments of
 // c are read, the copies of chunk c + 1 go out (after the reads, so that no
@@ -962,7 +975,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
 // while they arrive.
 __device__ __forceinline__ int seg_key(int r)
 {
-    return (r + (r >> 3)) & 7;
+    return r & 7;
 }
 
 template <int LK>
