@@ -658,11 +658,11 @@ __device__ __forceinline__ int row_atom(int s)
 // insertions run in a short divergent loop in increasing atom order.  Lane
 // (g, r16) holds rows 4 g + r of each block: atoms lane_atom0 + 4 r + 16 i
 // (of the sub-tile's first nvalid) against voxel w 64 + 16 v + r16.
-template <int NI, int LK>
-__device__ __forceinline__ void fold_subtile(const floatx4 (&acc_re)[NI][4],
-                                             const floatx4 (&acc_im)[NI][4], float (&th)[4],
-                                             float *cs, uint32_t *ci, int ah, int w, int g,
-                                             int r16, uint32_t lane_atom0, int nvalid)
+template <int NI, int LK, bool FULL>
+__device__ __forceinline__ void fold_subtile_body(const floatx4 (&acc_re)[NI][4],
+                                                  const floatx4 (&acc_im)[NI][4], float (&th)[4],
+                                                  float *cs, uint32_t *ci, int ah, int w, int g,
+                                                  int r16, uint32_t lane_atom0, int nvalid)
 {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -678,7 +678,8 @@ __device__ __forceinline__ void fold_subtile(const floatx4 (&acc_re)[NI][4],
                     const int i = 4 * h + ii, j = 4 * ii + r;
                     const float re = acc_re[i][v][r], im = acc_im[i][v][r];
                     sc[j] = re * re + im * im;
-                    if (g + 4 * r + 16 * (ah * NI + i) < nvalid && sc[j] > th[v]) m |= 1u << j;
+                    const bool valid = FULL || g + 4 * r + 16 * (ah * NI + i) < nvalid;
+                    if (valid && sc[j] > th[v]) m |= 1u << j;
                 }
             while (m) {
                 const int j = __builtin_ctz(m);
@@ -693,6 +694,23 @@ __device__ __forceinline__ void fold_subtile(const floatx4 (&acc_re)[NI][4],
             }
         }
     }
+}
+
+// a whole sub-tile (every row a valid atom) skips the per-score bound test
+// (the one-product kernel; the split kernels, at the edge of their
+// registers, keep the single form)
+template <int NI, int LK>
+__device__ __forceinline__ void fold_subtile(const floatx4 (&acc_re)[NI][4],
+                                             const floatx4 (&acc_im)[NI][4], float (&th)[4],
+                                             float *cs, uint32_t *ci, int ah, int w, int g,
+                                             int r16, uint32_t lane_atom0, int nvalid)
+{
+    if (nvalid >= LAT)
+        fold_subtile_body<NI, LK, true>(acc_re, acc_im, th, cs, ci, ah, w, g, r16, lane_atom0,
+                                        nvalid);
+    else
+        fold_subtile_body<NI, LK, false>(acc_re, acc_im, th, cs, ci, ah, w, g, r16, lane_atom0,
+                                         nvalid);
 }
 
 // KC timepoints per K chunk (16 or 32), NB LDS staging buffers (2: one
@@ -927,7 +945,7 @@ __global__ __launch_bounds__(kLowpThreads) void match_lowp_kernel(LowpArgs a)
             }
             __syncthreads();
         }
-        fold_subtile<NI, LK>(acc_re, acc_im, th, cs, ci, ah, w, g, r16,
+        fold_subtile_body<NI, LK, false>(acc_re, acc_im, th, cs, ci, ah, w, g, r16,
                              (uint32_t)(idx_base + sub) + (uint32_t)g, (int)nsub);
         sub += a_step;
     }
