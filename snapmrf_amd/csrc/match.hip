@@ -1250,6 +1250,19 @@ __global__ __launch_bounds__(kBlock) void top1_kernel(const float *part_s, const
     }
 }
 
+// lane l's value of x, on every lane (no LDS)
+__device__ __forceinline__ float wave_read(float x, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+__device__ __forceinline__ double wave_read(double x, int l)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 template <int K, int CK, typename Acc>
 __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(RescoreArgs ra)
 {
@@ -1341,9 +1354,9 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
     Acc bs = -1.0f;
     uint32_t bi = 0u;
     const float2 *x = sig + vox * (size_t)T;
-#pragma unroll
-    for (int c0 = 0; c0 < K; c0 += G) {
-        if (c0 >= nc) break;
+    // one group of G candidates from c0 (returns false past the last)
+    auto group = [&](int c0) {
+        if (c0 >= nc) return false;
         Acc re[G], im[G];
         const float2 *row[G];
 #pragma unroll
@@ -1383,16 +1396,43 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
                 }
             }
         }
+        // Sum the 2G lane partials over the wave as a reduce-scatter: each
+        // step over lane bit b halves the values a lane holds (the half of
+        // lanes with bit b set keeps the upper half) and adds the partner's
+        // copy, so after log2(2G) steps lane l holds value l >> (6 - S) (Re of
+        // candidate k for k < G, Im of candidate k - G) summed over the lanes
+        // that share its low 6 - S bits; those are then summed all-round.
+        // 2G + log2(2G) shuffles instead of 12 G.
+        constexpr int NV = 2 * G;
+        constexpr int S = NV == 32 ? 5 : NV == 16 ? 4 : 3;
+        static_assert((1 << S) == NV, "2G is 8, 16 or 32");
+        Acc v[NV];
 #pragma unroll
-        for (int c = 0; c < G; c++)
-            for (int off = 32; off >= 1; off >>= 1) {
-                re[c] += __shfl_xor(re[c], off);
-                im[c] += __shfl_xor(im[c], off);
+        for (int c = 0; c < G; c++) {
+            v[c] = re[c];
+            v[G + c] = im[c];
+        }
+#pragma unroll
+        for (int st = 0; st < S; st++) {
+            const int b = 5 - st;
+            const int n = NV >> (st + 1);
+            const bool upper = (lane >> b) & 1;
+#pragma unroll
+            for (int k = 0; k < n; k++) {
+                const Acc send = upper ? v[k] : v[k + n];
+                const Acc keep = upper ? v[k + n] : v[k];
+                v[k] = keep + __shfl_xor(send, 1 << b);
             }
+        }
+#pragma unroll
+        for (int b = 5 - S; b >= 0; b--) v[0] += __shfl_xor(v[0], 1 << b);
+        // lanes l < 32 hold Re of candidate l >> (6 - S), lane l + 32 its Im
+        const Acc vo = __shfl_xor(v[0], 32);
+        const Acc sc_lane = v[0] * v[0] + vo * vo;
 #pragma unroll
         for (int c = 0; c < G; c++)
             if (c0 + c < nc) {
-                const Acc sc = re[c] * re[c] + im[c] * im[c];
+                const Acc sc = wave_read(sc_lane, c << (6 - S));
                 uint32_t ci;
                 if constexpr (kCandLds) ci = cand_lds[threadIdx.x >> 6][c0 + c];
                 else ci = cand[c0 + c];
@@ -1401,6 +1441,17 @@ __global__ __launch_bounds__(kBlock, K > 32 ? 2 : 1) void rescore_kernel(Rescore
                     bi = ci;
                 }
             }
+        return true;
+    };
+    if constexpr (kCandLds) {
+        // candidates in LDS: a rolled loop keeps one group's registers live
+#pragma unroll 1
+        for (int c0 = 0; c0 < K; c0 += G)
+            if (!group(c0)) break;
+    } else {
+#pragma unroll
+        for (int c0 = 0; c0 < K; c0 += G)
+            if (!group(c0)) break;
     }
     double bsd = (double)bs;   // an FP32 score is exact in double
     if (nc == 0) {
